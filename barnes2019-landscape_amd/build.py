@@ -1,0 +1,73 @@
+"""Build recipe for the MI355X FastScape library (gfx950).
+
+Produces, in-tree under barnes2019-landscape_amd/lib/:
+  libfastscape_hip.so   the C-ABI library (include/fastscape.h)
+  fastscape_mi355x      the drop-in command line (main of fastscape_RB.cpp)
+
+Device code is compiled with -ffp-contract=off (no fused multiply-add: the
+reference's results depend on every rounding) and with packed FP32 ops
+disabled.  Usage: python barnes2019-landscape_amd/build.py
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB_DIR = os.path.join(HERE, "lib")
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(LIB_DIR, "libfastscape_hip.so")
+CLI = os.path.join(LIB_DIR, "fastscape_mi355x")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+OFFLOAD_ARCH = "gfx950"
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
+          "-Wno-unused-parameter"]
+DEVICE = ["--offload-arch=" + OFFLOAD_ARCH,
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _git_hash():
+    try:
+        return subprocess.run(["git", "-C", ROOT, "rev-parse", "HEAD"], check=True,
+                              capture_output=True, text=True).stdout.strip()
+    except Exception:
+        return "unknown"
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + " ".join(cmd[:3]) + " ...")
+    if verbose and (r.stdout or r.stderr):
+        # hipcc's host pass warns about the device-only target feature; keep
+        # the rest visible
+        for line in (r.stdout + r.stderr).splitlines():
+            if "packed-fp32-ops" not in line:
+                print(line)
+
+
+def build(verbose=False):
+    """Compile the library and the CLI (always recompiles)."""
+    if not os.path.exists(HIPCC):
+        raise RuntimeError("hipcc not found at " + HIPCC)
+    os.makedirs(LIB_DIR, exist_ok=True)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *COMMON, *DEVICE, "-shared", "-I" + INCLUDE, "-I" + CSRC,
+          os.path.join(CSRC, "fastscape_hip.hip"), os.path.join(CSRC, "fs_host.cpp"),
+          "-o", tmp], verbose)
+    os.replace(tmp, LIB)
+    cxx = shutil.which("g++") or "g++"
+    _run([cxx, "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE,
+          '-DGIT_HASH="%s"' % _git_hash(), os.path.join(CSRC, "fastscape_cli.cpp"),
+          "-L" + LIB_DIR, "-lfastscape_hip", "-Wl,-rpath,$ORIGIN", "-o", CLI], verbose)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

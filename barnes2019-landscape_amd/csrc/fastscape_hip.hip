@@ -1,0 +1,1340 @@
+// fastscape_hip.hip -- FastScape time-step on MI355X (gfx950).
+//
+// One time-step (the body of the reference's loop, fastscape_RB.cpp:366-376)
+// is a short chain of kernels on the model's stream:
+//
+//   k_receivers  D8 steepest-descent receiver of every interior cell
+//                (RB:160-182) -> rcv, one byte per cell
+//   k_topology   donor bitmask of every cell (the inverse of rcv, RB:188-203)
+//                and the level of every cell, i.e. its number of receiver
+//                hops to an outlet, with a per-level histogram; cells deeper
+//                than kWalkCap hops are finished by k_jump (pointer jumping)
+//   -- host reads the level histogram once per step, so every later launch
+//      is sized exactly --
+//   k_roots      level 0: every NO_FLOW cell in ascending index (RB:223-226),
+//                a single-pass stream compaction (decoupled look-back)
+//   k_bfs        level L+1 = donors of level L's cells in level-L order, each
+//                donor list in ascending cell index (RB:234-247), one
+//                single-pass scan per level; runs of small levels go to one
+//                workgroup (k_bfs_small), the analogue of RB+P's
+//                `if(lvlsize>500)` serial fallback (fastscape_RB+P.cpp:328)
+//   k_flowacc    upstream cell counts, pulled level by level from the top
+//                (pull form of RB:262-279, as fastscape_RB+PI.cpp:306-323)
+//   k_uplift     uplift of the interior outlets (pits) (RB:285-296); every
+//                other interior cell receives its uplift inside k_erode
+//   k_erode      implicit stream-power update of each level from level 1 up,
+//                the reference's Newton iteration replayed operation by
+//                operation (RB:306-328)
+//
+// Arithmetic is IEEE double with contraction off (-ffp-contract=off): the
+// slope is a true division, the erosion factor is (K*dt)*A^m / dr^n with
+// A^m read from a table filled by the host libm (the same pow the reference
+// calls), so receivers, levels, the stack and, for n = 1, elevations are
+// bit-identical to the reference's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "fs_internal.h"
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr uint8_t kNoFlow = 0xFF;
+constexpr int kWalkCap = 64;           // hops walked per cell in k_topology
+constexpr int kJumpPasses = 12;        // kWalkCap * 2^10 > FS_MAX_LEVELS
+constexpr int kScanBlock = 256;
+constexpr int kRootItems = 16;         // cells per thread in k_roots
+constexpr int kRootTile = kScanBlock * kRootItems;
+constexpr int kBfsItems = 8;           // stack entries per thread in k_bfs
+constexpr int kBfsTile = kScanBlock * kBfsItems;
+constexpr int kSmallBlock = 1024;
+constexpr int kSmallItems = 4;
+constexpr int kSmallLevel = kSmallBlock * kSmallItems;  // one-workgroup levels
+constexpr int kNewtonCap = 100;
+constexpr int kHdrInts = 4;            // maxdepth, n_unfinished, err, spare
+constexpr int kHdrCounts = 1024;       // level counts fetched with the header
+constexpr int kTopoBlocks = 2048;
+constexpr int kJumpBlocks = 1024;
+
+// device error bits (hdr[2])
+constexpr int kErrLookback = 1, kErrNewton = 2, kErrIndex = 4, kErrDepth = 8;
+
+// ---------------------------------------------------------------------------
+// neighbourhood encodings
+// ---------------------------------------------------------------------------
+// Receiver direction r = 0..7 is W, NW, N, NE, E, SE, S, SW (RB:79-82); its
+// offset is nshift[r] = {-1, -W-1, -W, -W+1, 1, W+1, W, W-1} (RB:137).
+__device__ __forceinline__ int dir_offset(int r, int W) {
+  constexpr unsigned kDx = (0u << 0) | (0u << 2) | (1u << 4) | (2u << 6) |
+                           (2u << 8) | (2u << 10) | (1u << 12) | (0u << 14);
+  constexpr unsigned kDy = (1u << 0) | (0u << 2) | (0u << 4) | (0u << 6) |
+                           (1u << 8) | (2u << 10) | (2u << 12) | (2u << 14);
+  const int dx = static_cast<int>((kDx >> (2 * r)) & 3u) - 1;
+  const int dy = static_cast<int>((kDy >> (2 * r)) & 3u) - 1;
+  return dy * W + dx;
+}
+
+// Donor bitmask slot j = 0..7 is the neighbour at offset
+// {-W-1, -W, -W+1, -1, +1, W-1, W, W+1}[j]: ascending cell index, so the set
+// bits in slot order list the donors in the order RB's push builds them
+// (RB:194-202).  The neighbour in slot j is a donor iff its receiver
+// direction is kSlotDir[j].
+__device__ __forceinline__ int slot_offset(int j, int W) {
+  constexpr unsigned kSx = (0u << 0) | (1u << 2) | (2u << 4) | (0u << 6) |
+                           (2u << 8) | (0u << 10) | (1u << 12) | (2u << 14);
+  constexpr unsigned kSy = (0u << 0) | (0u << 2) | (0u << 4) | (1u << 6) |
+                           (1u << 8) | (2u << 10) | (2u << 12) | (2u << 14);
+  const int dx = static_cast<int>((kSx >> (2 * j)) & 3u) - 1;
+  const int dy = static_cast<int>((kSy >> (2 * j)) & 3u) - 1;
+  return dy * W + dx;
+}
+
+// ---------------------------------------------------------------------------
+// scans
+// ---------------------------------------------------------------------------
+// Exclusive prefix sum over the block; *total gets the block sum.  sh holds
+// BLOCK/64 + 1 words.  Ends with a barrier, so sh may be reused at once.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total,
+                                                    uint32_t *sh) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == kWave - 1) sh[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < BLOCK / kWave; w++) {
+      const uint32_t t = sh[w];
+      sh[w] = run;
+      run += t;
+    }
+    sh[BLOCK / kWave] = run;
+  }
+  __syncthreads();
+  const uint32_t ex = sh[wid] + x - v;
+  *total = sh[BLOCK / kWave];
+  __syncthreads();
+  return ex;
+}
+
+// Tile status word: [epoch:30 | flag:2 | value:32].  flag 1 = tile aggregate,
+// 2 = inclusive prefix.  A word whose epoch is not the launch's epoch is
+// "not yet published", so the array never needs clearing.
+__device__ __forceinline__ uint64_t status_word(uint32_t epoch, uint32_t flag, uint32_t v) {
+  return (static_cast<uint64_t>(epoch & 0x3fffffffu) << 34) |
+         (static_cast<uint64_t>(flag) << 32) | v;
+}
+
+// Decoupled look-back (Merrill & Garland, "Single-pass Parallel Prefix Scan
+// with Decoupled Look-back", 2016), run by one full wave.  Publishes the
+// tile's aggregate, sums predecessors' aggregates 64 at a time until an
+// inclusive prefix is found, publishes its own inclusive prefix and returns
+// the tile's exclusive prefix in every lane.  Tiles are numbered by an atomic
+// ticket, so every predecessor is resident and publishes its aggregate
+// without waiting; the wait is still bounded and reported in *err.
+__device__ uint32_t lookback(uint64_t *status, uint32_t tile, uint32_t epoch,
+                             uint32_t agg, int32_t *err) {
+  const int lane = threadIdx.x & (kWave - 1);
+  epoch &= 0x3fffffffu;
+  if (tile == 0) {
+    if (lane == 0)
+      __hip_atomic_store(&status[0], status_word(epoch, 2, agg), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    return 0;
+  }
+  if (lane == 0)
+    __hip_atomic_store(&status[tile], status_word(epoch, 1, agg), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t excl = 0;
+  long long base = static_cast<long long>(tile) - 1;
+  int spins = 0;
+  while (true) {
+    const long long idx = base - lane;
+    uint64_t w = status_word(epoch, 2, 0);
+    if (idx >= 0)
+      w = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t flag = static_cast<uint32_t>(w >> 32) & 3u;
+    const bool valid = (static_cast<uint32_t>(w >> 34) == epoch) && flag != 0;
+    const unsigned long long inval = __ballot(!valid);
+    const unsigned long long pmask = __ballot(valid && flag == 2);
+    const int first_p = pmask ? __builtin_ctzll(pmask) : kWave;
+    const int first_inv = inval ? __builtin_ctzll(inval) : kWave;
+    if (first_inv < first_p) {
+      if (++spins > (1 << 22)) {
+        if (lane == 0) atomicOr(err, kErrLookback);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    uint32_t v = (lane <= first_p) ? static_cast<uint32_t>(w) : 0u;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    excl += v;
+    if (first_p < kWave) break;
+    base -= kWave;
+  }
+  if (lane == 0)
+    __hip_atomic_store(&status[tile], status_word(epoch, 2, excl + agg),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return excl;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void k_fill_counts(int32_t *hdr, int n_counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    hdr[0] = 0;   // maxdepth
+    hdr[1] = 0;   // n_unfinished
+  }
+  if (i < n_counts) hdr[kHdrInts + i] = 0;
+}
+
+// ComputeReceivers (RB:160-182).  64x4 threads per block, one interior cell
+// each; neighbour rows are shared through L1/L2.  Directions are tried in
+// order 0..7 with a strict '>' from 0, so ties keep the lowest direction and
+// flat cells and pits stay NO_FLOW.
+__global__ __launch_bounds__(256) void k_receivers(const double *__restrict__ h,
+                                                   uint8_t *__restrict__ rcv, int W,
+                                                   int H, double d_card, double d_diag) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
+  if (x >= W - 2) return;
+  for (int y = blockIdx.y * 4 + (threadIdx.x >> 6) + 2; y < H - 2; y += gridDim.y * 4) {
+    const size_t c = static_cast<size_t>(y) * W + x;
+    const double *up = h + c - W;
+    const double *dn = h + c + W;
+    const double hc = h[c];
+    double ms = 0.0, s;
+    int mn = kNoFlow;
+    s = (hc - h[c - 1]) / d_card; if (s > ms) { ms = s; mn = 0; }
+    s = (hc - up[-1]) / d_diag;   if (s > ms) { ms = s; mn = 1; }
+    s = (hc - up[0]) / d_card;    if (s > ms) { ms = s; mn = 2; }
+    s = (hc - up[1]) / d_diag;    if (s > ms) { ms = s; mn = 3; }
+    s = (hc - h[c + 1]) / d_card; if (s > ms) { ms = s; mn = 4; }
+    s = (hc - dn[1]) / d_diag;    if (s > ms) { ms = s; mn = 5; }
+    s = (hc - dn[0]) / d_card;    if (s > ms) { ms = s; mn = 6; }
+    s = (hc - dn[-1]) / d_diag;   if (s > ms) { ms = s; mn = 7; }
+    rcv[c] = static_cast<uint8_t>(mn);
+  }
+}
+
+// Donor bitmask and level of every cell.  Rows are strided over the grid so
+// each block folds its level histogram in LDS before touching HBM.  The
+// level is found by walking receivers (all reads hit L1/L2: the receiver
+// field is one byte per cell and the walk moves one cell per hop); walks
+// longer than kWalkCap hops record (cell reached, hops) for k_jump.
+__global__ __launch_bounds__(256) void k_topology(
+    const uint8_t *__restrict__ rcv, uint8_t *__restrict__ dmask,
+    uint16_t *__restrict__ depth, unsigned long long *__restrict__ jump,
+    int32_t *__restrict__ list, int32_t *__restrict__ hdr, int W, int H) {
+  __shared__ int hist[kWalkCap + 1];
+  __shared__ int smax;
+  for (int i = threadIdx.x; i <= kWalkCap; i += blockDim.x) hist[i] = 0;
+  if (threadIdx.x == 0) smax = 0;
+  __syncthreads();
+  int32_t *counts = hdr + kHdrInts;
+  int local_max = 0;
+  for (int y = blockIdx.x; y < H; y += gridDim.x) {
+    const bool yin = y > 0 && y < H - 1;
+    for (int x = threadIdx.x; x < W; x += blockDim.x) {
+      const int c = y * W + x;
+      uint32_t m = 0;
+      if (yin && x > 0 && x < W - 1) {
+        const uint8_t *up = rcv + c - W;
+        const uint8_t *dn = rcv + c + W;
+        m |= (up[-1] == 5) ? 0x01u : 0u;
+        m |= (up[0] == 6) ? 0x02u : 0u;
+        m |= (up[1] == 7) ? 0x04u : 0u;
+        m |= (rcv[c - 1] == 4) ? 0x08u : 0u;
+        m |= (rcv[c + 1] == 0) ? 0x10u : 0u;
+        m |= (dn[-1] == 3) ? 0x20u : 0u;
+        m |= (dn[0] == 2) ? 0x40u : 0u;
+        m |= (dn[1] == 1) ? 0x80u : 0u;
+      }
+      dmask[c] = static_cast<uint8_t>(m);
+      int cur = c, d = 0;
+      int r = rcv[c];
+      while (r < 8 && d < kWalkCap) {
+        cur += dir_offset(r, W);
+        d++;
+        r = rcv[cur];
+      }
+      if (r >= 8) {
+        depth[c] = static_cast<uint16_t>(d);
+        atomicAdd(&hist[d], 1);
+        local_max = max(local_max, d);
+      } else {
+        depth[c] = 0xFFFF;
+        jump[c] = (static_cast<unsigned long long>(d) << 32) | static_cast<uint32_t>(cur);
+        const int slot = atomicAdd(&hdr[1], 1);
+        list[slot] = c;
+      }
+    }
+  }
+  atomicMax(&smax, local_max);
+  __syncthreads();
+  for (int i = threadIdx.x; i <= kWalkCap; i += blockDim.x)
+    if (hist[i]) atomicAdd(&counts[i], hist[i]);
+  if (threadIdx.x == 0) atomicMax(&hdr[0], smax);
+}
+
+// One pointer-jumping pass over the cells k_topology left unfinished: a cell
+// whose recorded ancestor has a level takes level = hops + that level;
+// otherwise it jumps to its ancestor's ancestor.  (ancestor, hops) pairs are
+// read and written as one 64-bit word, so every pair a thread sees is
+// consistent and the update is safe in place.
+__global__ __launch_bounds__(256) void k_jump(const int32_t *__restrict__ list,
+                                              uint16_t *depth, unsigned long long *jump,
+                                              int32_t *hdr) {
+  const int n = hdr[1];
+  int32_t *counts = hdr + kHdrInts;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = list[i];
+    if (__hip_atomic_load(&depth[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0xFFFF)
+      continue;
+    const unsigned long long j =
+        __hip_atomic_load(&jump[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int a = static_cast<int>(j & 0xffffffffu);
+    const unsigned d = static_cast<unsigned>(j >> 32);
+    const unsigned da = __hip_atomic_load(&depth[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (da != 0xFFFF) {
+      const unsigned nd = d + da;
+      if (nd > FS_MAX_LEVELS) {
+        atomicOr(&hdr[2], kErrDepth);
+        continue;
+      }
+      __hip_atomic_store(&depth[c], static_cast<uint16_t>(nd), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      atomicAdd(&counts[nd], 1);
+      atomicMax(&hdr[0], static_cast<int>(nd));
+    } else {
+      const unsigned long long ja =
+          __hip_atomic_load(&jump[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned nd = d + static_cast<unsigned>(ja >> 32);
+      if (nd > FS_MAX_LEVELS + 1u) {
+        atomicOr(&hdr[2], kErrDepth);
+        continue;
+      }
+      __hip_atomic_store(&jump[c],
+                         (static_cast<unsigned long long>(nd) << 32) | (ja & 0xffffffffu),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// Level 0 of GenerateOrder (RB:223-226): every NO_FLOW cell, ring 0
+// included, in ascending index.  One tile = 4096 cells, 16 per thread read
+// as one 16-byte load.
+__global__ __launch_bounds__(kScanBlock) void k_roots(
+    const uint8_t *__restrict__ rcv, int32_t *__restrict__ stack, uint64_t *status,
+    unsigned long long *ticket, unsigned long long tbase, uint32_t epoch, int32_t N,
+    int32_t *hdr) {
+  __shared__ uint32_t sh[kScanBlock / kWave + 1];
+  __shared__ uint32_t s_tile, s_pre;
+  if (threadIdx.x == 0) s_tile = static_cast<uint32_t>(atomicAdd(ticket, 1ull) - tbase);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const long long base = static_cast<long long>(tile) * kRootTile +
+                         static_cast<long long>(threadIdx.x) * kRootItems;
+  uint32_t flags = 0;
+  if (base + kRootItems <= N) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(rcv + base);
+    const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+        if (((w4[k] >> (8 * b)) & 0xFFu) == kNoFlow) flags |= 1u << (4 * k + b);
+  } else {
+    for (int i = 0; i < kRootItems; i++)
+      if (base + i < N && rcv[base + i] == kNoFlow) flags |= 1u << i;
+  }
+  uint32_t total;
+  const uint32_t ex = block_excl_scan<kScanBlock>(__popc(flags), &total, sh);
+  if (threadIdx.x < kWave) {
+    const uint32_t pre = lookback(status, tile, epoch, total, &hdr[2]);
+    if (threadIdx.x == 0) s_pre = pre;
+  }
+  __syncthreads();
+  long long pos = static_cast<long long>(s_pre) + ex;
+  while (flags) {
+    const int i = __builtin_ctz(flags);
+    flags &= flags - 1;
+    if (pos < N) stack[pos] = static_cast<int32_t>(base + i);
+    pos++;
+  }
+}
+
+// One BFS level of GenerateOrder (RB:234-247): stack[ls, le) -> the donor
+// lists of those cells, in order, written from stack[le].
+__global__ __launch_bounds__(kScanBlock) void k_bfs(
+    int32_t *__restrict__ stack, const uint8_t *__restrict__ dmask, int32_t ls, int32_t le,
+    int W, int32_t N, uint64_t *status, unsigned long long *ticket,
+    unsigned long long tbase, uint32_t epoch, int32_t *hdr) {
+  __shared__ uint32_t sh[kScanBlock / kWave + 1];
+  __shared__ uint32_t s_tile, s_pre;
+  if (threadIdx.x == 0) s_tile = static_cast<uint32_t>(atomicAdd(ticket, 1ull) - tbase);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const long long p0 = static_cast<long long>(ls) + static_cast<long long>(tile) * kBfsTile +
+                       static_cast<long long>(threadIdx.x) * kBfsItems;
+  int32_t cell[kBfsItems];
+  unsigned long long masks = 0;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int i = 0; i < kBfsItems; i++) {
+    const long long p = p0 + i;
+    int32_t c = 0;
+    uint32_t m = 0;
+    if (p < le) {
+      c = stack[p];
+      if (static_cast<uint32_t>(c) < static_cast<uint32_t>(N)) m = dmask[c];
+      else atomicOr(&hdr[2], kErrIndex);
+    }
+    cell[i] = c;
+    masks |= static_cast<unsigned long long>(m) << (8 * i);
+    cnt += __popc(m);
+  }
+  uint32_t total;
+  const uint32_t ex = block_excl_scan<kScanBlock>(cnt, &total, sh);
+  if (threadIdx.x < kWave) {
+    const uint32_t pre = lookback(status, tile, epoch, total, &hdr[2]);
+    if (threadIdx.x == 0) s_pre = pre;
+  }
+  __syncthreads();
+  long long pos = static_cast<long long>(le) + s_pre + ex;
+#pragma unroll
+  for (int i = 0; i < kBfsItems; i++) {
+    uint32_t m = static_cast<uint32_t>(masks >> (8 * i)) & 0xFFu;
+    while (m) {
+      const int j = __builtin_ctz(m);
+      m &= m - 1;
+      if (pos < N) stack[pos] = cell[i] + slot_offset(j, W);
+      pos++;
+    }
+  }
+}
+
+// Levels [L0, L1) of the BFS in one workgroup, for runs of small levels.
+__global__ __launch_bounds__(kSmallBlock) void k_bfs_small(
+    int32_t *__restrict__ stack, const uint8_t *__restrict__ dmask,
+    const int32_t *__restrict__ levels, int L0, int L1, int W, int32_t N, int32_t *hdr) {
+  __shared__ uint32_t sh[kSmallBlock / kWave + 1];
+  for (int L = L0; L < L1; L++) {
+    const int32_t ls = levels[L], le = levels[L + 1];
+    long long out = le;
+    for (long long b = ls; b < le; b += kSmallLevel) {
+      const long long p0 = b + static_cast<long long>(threadIdx.x) * kSmallItems;
+      int32_t cell[kSmallItems];
+      uint32_t masks = 0, cnt = 0;
+#pragma unroll
+      for (int i = 0; i < kSmallItems; i++) {
+        const long long p = p0 + i;
+        int32_t c = 0;
+        uint32_t m = 0;
+        if (p < le) {
+          c = stack[p];
+          if (static_cast<uint32_t>(c) < static_cast<uint32_t>(N)) m = dmask[c];
+          else atomicOr(&hdr[2], kErrIndex);
+        }
+        cell[i] = c;
+        masks |= m << (8 * i);
+        cnt += __popc(m);
+      }
+      uint32_t total;
+      const uint32_t ex = block_excl_scan<kSmallBlock>(cnt, &total, sh);
+      long long pos = out + ex;
+#pragma unroll
+      for (int i = 0; i < kSmallItems; i++) {
+        uint32_t m = (masks >> (8 * i)) & 0xFFu;
+        while (m) {
+          const int j = __builtin_ctz(m);
+          m &= m - 1;
+          if (pos < N) stack[pos] = cell[i] + slot_offset(j, W);
+          pos++;
+        }
+      }
+      out += total;
+    }
+    __syncthreads();
+  }
+}
+
+// Upstream cell count of one cell: 1 + the counts of its donors, which sit
+// one level higher and are already final.  accum = count * cell_area exactly
+// (RB:262-279: every partial sum there is an integer multiple of cell_area).
+__device__ __forceinline__ void flowacc_cell(int32_t c, const uint8_t *__restrict__ dmask,
+                                             uint32_t *cnt, int W, int32_t N, int32_t *hdr) {
+  if (static_cast<uint32_t>(c) >= static_cast<uint32_t>(N)) {
+    atomicOr(&hdr[2], kErrIndex);
+    return;
+  }
+  uint32_t m = dmask[c];
+  uint32_t s = 1;
+  while (m) {
+    const int j = __builtin_ctz(m);
+    m &= m - 1;
+    s += cnt[c + slot_offset(j, W)];
+  }
+  cnt[c] = s;
+}
+
+__global__ __launch_bounds__(256) void k_flowacc(const int32_t *__restrict__ stack,
+                                                 const uint8_t *__restrict__ dmask,
+                                                 uint32_t *cnt, int32_t ls, int32_t le, int W,
+                                                 int32_t N, int32_t *hdr) {
+  const long long p = static_cast<long long>(ls) + blockIdx.x * 256ll + threadIdx.x;
+  if (p >= le) return;
+  flowacc_cell(stack[p], dmask, cnt, W, N, hdr);
+}
+
+// Levels L_hi down to L_lo (inclusive) in one workgroup.
+__global__ __launch_bounds__(kSmallBlock) void k_flowacc_small(
+    const int32_t *__restrict__ stack, const uint8_t *__restrict__ dmask, uint32_t *cnt,
+    const int32_t *__restrict__ levels, int L_hi, int L_lo, int W, int32_t N, int32_t *hdr) {
+  for (int L = L_hi; L >= L_lo; L--) {
+    const int32_t ls = levels[L], le = levels[L + 1];
+    for (long long p = ls + threadIdx.x; p < le; p += kSmallBlock)
+      flowacc_cell(stack[p], dmask, cnt, W, N, hdr);
+    __syncthreads();
+  }
+}
+
+// AddUplift (RB:285-296) for the interior outlets (level-0 cells away from
+// rings 0 and 1).  Interior cells with a receiver are uplifted in k_erode.
+__global__ __launch_bounds__(256) void k_uplift_roots(const int32_t *__restrict__ stack,
+                                                      int32_t n0, double *h, int W, int H,
+                                                      double ud, int32_t N, int32_t *hdr) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n0) return;
+  const int32_t c = stack[p];
+  if (static_cast<uint32_t>(c) >= static_cast<uint32_t>(N)) {
+    atomicOr(&hdr[2], kErrIndex);
+    return;
+  }
+  const int y = c / W, x = c - y * W;
+  if (x >= 2 && x < W - 2 && y >= 2 && y < H - 2) h[c] += ud;
+}
+
+struct ErodeConst {
+  double kdt;       // keq*dt
+  double lenpow0;   // pow(dx, neq)        cardinal receiver
+  double lenpow1;   // pow(dx*SQRT2, neq)  diagonal receiver
+  double ud;        // ueq*dt
+  double neq;
+  double tol;
+};
+
+// Erode for one cell (RB:306-328).  NMODE 1: n = 1, pow(x,1) = x and
+// pow(x,0) = 1 exactly as libm returns them; NMODE 2: n = 2 with x*x for
+// pow(x,2); NMODE 0: any n through the device pow.  h0 is the uplifted
+// elevation h + ueq*dt (RB:294), hn the receiver's final elevation (its
+// level was processed earlier).
+template <int NMODE>
+__device__ __forceinline__ void erode_cell(int32_t c, const uint8_t *__restrict__ rcv,
+                                           const uint32_t *__restrict__ cnt,
+                                           const double *__restrict__ powtab, double *h,
+                                           int W, int32_t N, const ErodeConst &k,
+                                           int32_t *hdr) {
+  if (static_cast<uint32_t>(c) >= static_cast<uint32_t>(N)) {
+    atomicOr(&hdr[2], kErrIndex);
+    return;
+  }
+  const int r = rcv[c];
+  if (r >= 8) return;
+  const int32_t n = c + dir_offset(r, W);
+  uint32_t a = cnt[c];
+  if (a > static_cast<uint32_t>(N)) {
+    atomicOr(&hdr[2], kErrIndex);
+    a = static_cast<uint32_t>(N);
+  }
+  const double fact = (k.kdt * powtab[a]) / ((r & 1) ? k.lenpow1 : k.lenpow0);
+  const double h0 = h[c] + k.ud;
+  const double hn = h[n];
+  double hnew = h0, hp = h0, diff = 2.0 * k.tol;
+  int it = 0;
+  while (fabs(diff) > k.tol) {
+    const double x = hnew - hn;
+    double p1, p0;
+    if (NMODE == 1) {
+      p1 = x;
+      p0 = 1.0;
+    } else if (NMODE == 2) {
+      p1 = x * x;
+      p0 = x;
+    } else {
+      p1 = pow(x, k.neq);
+      p0 = pow(x, k.neq - 1.0);
+    }
+    hnew = hnew - ((hnew - h0) + fact * p1) / (1.0 + (fact * k.neq) * p0);
+    diff = hnew - hp;
+    hp = hnew;
+    if (++it >= kNewtonCap) {
+      atomicOr(&hdr[2], kErrNewton);
+      break;
+    }
+  }
+  h[c] = hnew;
+}
+
+template <int NMODE>
+__global__ __launch_bounds__(256) void k_erode(const int32_t *__restrict__ stack,
+                                               const uint8_t *__restrict__ rcv,
+                                               const uint32_t *__restrict__ cnt,
+                                               const double *__restrict__ powtab, double *h,
+                                               int32_t ls, int32_t le, int W, int32_t N,
+                                               ErodeConst k, int32_t *hdr) {
+  const long long p = static_cast<long long>(ls) + blockIdx.x * 256ll + threadIdx.x;
+  if (p >= le) return;
+  erode_cell<NMODE>(stack[p], rcv, cnt, powtab, h, W, N, k, hdr);
+}
+
+template <int NMODE>
+__global__ __launch_bounds__(kSmallBlock) void k_erode_small(
+    const int32_t *__restrict__ stack, const uint8_t *__restrict__ rcv,
+    const uint32_t *__restrict__ cnt, const double *__restrict__ powtab, double *h,
+    const int32_t *__restrict__ levels, int L_lo, int L_hi, int W, int32_t N, ErodeConst k,
+    int32_t *hdr) {
+  for (int L = L_lo; L <= L_hi; L++) {
+    const int32_t ls = levels[L], le = levels[L + 1];
+    for (long long p = ls + threadIdx.x; p < le; p += kSmallBlock)
+      erode_cell<NMODE>(stack[p], rcv, cnt, powtab, h, W, N, k, hdr);
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// host side
+// ===========================================================================
+#define FS_HIP_TRY(expr)                                                           \
+  do {                                                                             \
+    const hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess) {                                                        \
+      fs::set_last_error(std::string(#expr) + ": " + hipGetErrorString(e_));       \
+      return FS_ERR_HIP;                                                           \
+    }                                                                              \
+  } while (0)
+
+#define FS_TRY(expr)              \
+  do {                            \
+    const int s_ = (expr);        \
+    if (s_ != FS_OK) return s_;   \
+  } while (0)
+
+struct KTimed {
+  int kid;
+  hipEvent_t a, b;
+};
+
+struct fs_model {
+  int W = 0, H = 0;
+  int32_t N = 0;
+  int device = 0;
+  fs_params p{};
+  double d_card = 0, d_diag = 0;
+  ErodeConst ek{};
+  int nmode = 1;
+  hipStream_t stream = nullptr;
+
+  // device state
+  double *h = nullptr;
+  uint8_t *rcv = nullptr, *dmask = nullptr;
+  uint16_t *depth = nullptr;
+  unsigned long long *jump = nullptr;
+  int32_t *list = nullptr, *stack = nullptr, *hdr = nullptr, *levels_d = nullptr;
+  uint32_t *cnt = nullptr;
+  double *powtab = nullptr;
+  uint64_t *status = nullptr;
+  unsigned long long *ticket = nullptr;
+
+  // host mirrors
+  int32_t *hdr_h = nullptr;      // pinned: header + level counts
+  int32_t *levels_h = nullptr;   // pinned: levels uploaded each step
+  std::vector<int32_t> counts, levels;
+  int32_t nlevel = 0, maxdepth = 0, prev_maxdepth = FS_MAX_LEVELS;
+  bool have_step = false;
+  unsigned long long ticket_base = 0;
+  uint32_t epoch = 0;
+
+  // timers
+  uint64_t timer_us[FS_NUM_TIMERS] = {};
+  double stage_ms[FS_NUM_TIMERS] = {};
+  hipEvent_t ev[2][7] = {};
+  bool ev_pending[2] = {false, false};
+  int ev_cur = 0;
+
+  bool ktiming = false;
+  std::vector<KTimed> kpending;
+  std::vector<hipEvent_t> kfree;
+  double kms[FS_NUM_KERNELS] = {};
+  uint64_t klaunches[FS_NUM_KERNELS] = {};
+};
+
+namespace {
+
+uint32_t next_epoch(fs_model *m) {
+  m->epoch = (m->epoch + 1) & 0x3fffffffu;
+  if (m->epoch == 0) m->epoch = 1;
+  return m->epoch;
+}
+
+int kt_begin(fs_model *m, int kid) {
+  if (!m->ktiming) return FS_OK;
+  KTimed t{kid, nullptr, nullptr};
+  for (hipEvent_t *e : {&t.a, &t.b}) {
+    if (!m->kfree.empty()) {
+      *e = m->kfree.back();
+      m->kfree.pop_back();
+    } else {
+      FS_HIP_TRY(hipEventCreate(e));
+    }
+  }
+  FS_HIP_TRY(hipEventRecord(t.a, m->stream));
+  m->kpending.push_back(t);
+  return FS_OK;
+}
+
+int kt_end(fs_model *m) {
+  if (!m->ktiming) return FS_OK;
+  FS_HIP_TRY(hipEventRecord(m->kpending.back().b, m->stream));
+  return FS_OK;
+}
+
+// All recorded kernel events are complete (called after a stream sync).
+int kt_collect(fs_model *m) {
+  for (auto &t : m->kpending) {
+    float ms = 0.f;
+    FS_HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
+    m->kms[t.kid] += ms;
+    m->klaunches[t.kid]++;
+    m->kfree.push_back(t.a);
+    m->kfree.push_back(t.b);
+  }
+  m->kpending.clear();
+  return FS_OK;
+}
+
+int stage_collect(fs_model *m, int set) {
+  if (!m->ev_pending[set]) return FS_OK;
+  for (int i = 0; i < 6; i++) {
+    float ms = 0.f;
+    FS_HIP_TRY(hipEventElapsedTime(&ms, m->ev[set][i], m->ev[set][i + 1]));
+    m->stage_ms[FS_T_RECEIVERS + i] += ms;
+  }
+  m->ev_pending[set] = false;
+  return FS_OK;
+}
+
+int check_launch() {
+  FS_HIP_TRY(hipGetLastError());
+  return FS_OK;
+}
+
+int device_error_status(int32_t err) {
+  if (err & kErrNewton) return FS_ERR_NEWTON;
+  if (err & kErrDepth) return FS_ERR_LEVEL_OVERFLOW;
+  if (err) {
+    fs::set_last_error("device consistency flag " + std::to_string(err));
+    return FS_ERR_INTERNAL;
+  }
+  return FS_OK;
+}
+
+void free_model(fs_model *m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  for (void *ptr : {static_cast<void *>(m->h), static_cast<void *>(m->rcv),
+                    static_cast<void *>(m->dmask), static_cast<void *>(m->depth),
+                    static_cast<void *>(m->jump), static_cast<void *>(m->list),
+                    static_cast<void *>(m->stack), static_cast<void *>(m->hdr),
+                    static_cast<void *>(m->levels_d), static_cast<void *>(m->cnt),
+                    static_cast<void *>(m->powtab), static_cast<void *>(m->status),
+                    static_cast<void *>(m->ticket)})
+    if (ptr) (void)hipFree(ptr);
+  if (m->hdr_h) (void)hipHostFree(m->hdr_h);
+  if (m->levels_h) (void)hipHostFree(m->levels_h);
+  for (auto &set : m->ev)
+    for (auto &e : set)
+      if (e) (void)hipEventDestroy(e);
+  for (auto &t : m->kpending) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (auto e : m->kfree) (void)hipEventDestroy(e);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+template <class T>
+int dev_alloc(T **ptr, size_t count) {
+  const hipError_t e = hipMalloc(reinterpret_cast<void **>(ptr), std::max<size_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    fs::set_last_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return FS_ERR_OUT_OF_MEMORY;
+  }
+  return FS_OK;
+}
+
+int validate(int W, int H, const fs_params *p) {
+  if (!p) return FS_ERR_INVALID_ARGUMENT;
+  if (W < 5 || H < 5 || static_cast<long long>(W) * H >= (1ll << 31)) return FS_ERR_DIMENSIONS;
+  if (!(p->dx > 0) || !(p->cell_area > 0) || !(p->tol > 0) || !std::isfinite(p->keq) ||
+      !std::isfinite(p->neq) || !std::isfinite(p->meq) || !std::isfinite(p->ueq) ||
+      !std::isfinite(p->dt))
+    return FS_ERR_INVALID_ARGUMENT;
+  // accum = count*cell_area must stay an exact integer-valued double
+  if (static_cast<double>(W) * H * p->cell_area > 9007199254740992.0) return FS_ERR_INVALID_ARGUMENT;
+  return FS_OK;
+}
+
+int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_t seed,
+                 int device, fs_model **out) {
+  if (!out) return FS_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  FS_TRY(validate(W, H, p));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev) {
+    fs::set_last_error("no HIP device " + std::to_string(device));
+    return FS_ERR_NO_DEVICE;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  fs_model *m = new (std::nothrow) fs_model();
+  if (!m) return FS_ERR_OUT_OF_MEMORY;
+  struct Guard {
+    fs_model *m;
+    ~Guard() { if (m) free_model(m); }
+  } guard{m};
+  m->W = W;
+  m->H = H;
+  m->N = W * H;
+  m->device = device;
+  m->p = *p;
+  m->d_card = p->dx;
+  m->d_diag = p->dx * fs::kSqrt2;
+  m->ek.kdt = p->keq * p->dt;
+  m->ek.lenpow0 = std::pow(m->d_card, p->neq);
+  m->ek.lenpow1 = std::pow(m->d_diag, p->neq);
+  m->ek.ud = p->ueq * p->dt;
+  m->ek.neq = p->neq;
+  m->ek.tol = p->tol;
+  m->nmode = (p->neq == 1.0) ? 1 : (p->neq == 2.0) ? 2 : 0;
+  FS_HIP_TRY(hipSetDevice(device));
+  FS_HIP_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+  for (auto &set : m->ev)
+    for (auto &e : set) FS_HIP_TRY(hipEventCreate(&e));
+
+  const size_t N = static_cast<size_t>(m->N);
+  const size_t ntiles = (N + kBfsTile - 1) / kBfsTile + 1;
+  FS_TRY(dev_alloc(&m->h, N));
+  FS_TRY(dev_alloc(&m->rcv, N + 16));
+  FS_TRY(dev_alloc(&m->dmask, N));
+  FS_TRY(dev_alloc(&m->depth, N));
+  FS_TRY(dev_alloc(&m->jump, N));
+  FS_TRY(dev_alloc(&m->list, N));
+  FS_TRY(dev_alloc(&m->stack, N));
+  FS_TRY(dev_alloc(&m->cnt, N));
+  FS_TRY(dev_alloc(&m->powtab, N + 1));
+  FS_TRY(dev_alloc(&m->hdr, kHdrInts + FS_MAX_LEVELS + 3));
+  FS_TRY(dev_alloc(&m->levels_d, FS_MAX_LEVELS + 4));
+  FS_TRY(dev_alloc(&m->status, ntiles));
+  FS_TRY(dev_alloc(&m->ticket, 1));
+  if (hipHostMalloc(reinterpret_cast<void **>(&m->hdr_h),
+                    (kHdrInts + FS_MAX_LEVELS + 3) * sizeof(int32_t)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&m->levels_h),
+                    (FS_MAX_LEVELS + 4) * sizeof(int32_t)) != hipSuccess) {
+    fs::set_last_error("hipHostMalloc failed");
+    return FS_ERR_OUT_OF_MEMORY;
+  }
+  FS_HIP_TRY(hipMemsetAsync(m->rcv, 0xFF, N + 16, m->stream));   // RB:361-362
+  FS_HIP_TRY(hipMemsetAsync(m->hdr, 0, (kHdrInts + FS_MAX_LEVELS + 3) * sizeof(int32_t), m->stream));
+  FS_HIP_TRY(hipMemsetAsync(m->status, 0, ntiles * sizeof(uint64_t), m->stream));
+  FS_HIP_TRY(hipMemsetAsync(m->ticket, 0, sizeof(unsigned long long), m->stream));
+  FS_HIP_TRY(hipMemsetAsync(m->cnt, 0, N * sizeof(uint32_t), m->stream));
+  {
+    std::vector<double> tab;
+    try {
+      tab.resize(N + 1);
+    } catch (const std::bad_alloc &) {
+      return FS_ERR_OUT_OF_MEMORY;
+    }
+    fs::fill_pow_table(tab.data(), N, p->cell_area, p->meq);
+    FS_HIP_TRY(hipMemcpy(m->powtab, tab.data(), (N + 1) * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (h_init) {
+    FS_HIP_TRY(hipMemcpy(m->h, h_init, N * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<double> h0;
+    try {
+      h0.resize(N);
+    } catch (const std::bad_alloc &) {
+      return FS_ERR_OUT_OF_MEMORY;
+    }
+    fs::generate_terrain(h0.data(), W, H, seed);
+    FS_HIP_TRY(hipMemcpy(m->h, h0.data(), N * sizeof(double), hipMemcpyHostToDevice));
+  }
+  FS_HIP_TRY(hipStreamSynchronize(m->stream));
+  const auto us = std::chrono::duration_cast<std::chrono::microseconds>(
+                      std::chrono::steady_clock::now() - t0).count();
+  m->timer_us[FS_T_INITIALIZE] += static_cast<uint64_t>(us);
+  m->timer_us[FS_T_OVERALL] += static_cast<uint64_t>(us);
+  guard.m = nullptr;
+  *out = m;
+  return FS_OK;
+}
+
+// Read the header (max level, unfinished count, error bits) and the level
+// histogram; synchronises the stream.
+int fetch_header(fs_model *m, int ncounts) {
+  ncounts = std::min(ncounts, FS_MAX_LEVELS + 3);
+  FS_HIP_TRY(hipMemcpyAsync(m->hdr_h, m->hdr, (kHdrInts + ncounts) * sizeof(int32_t),
+                            hipMemcpyDeviceToHost, m->stream));
+  FS_HIP_TRY(hipStreamSynchronize(m->stream));
+  return FS_OK;
+}
+
+// Plan a level sweep: maximal runs of levels whose size is at most
+// kSmallLevel go to one single-workgroup launch; every larger level gets its
+// own grid-wide launch.
+struct Span {
+  int lo, hi;   // inclusive level range
+  bool small;
+};
+
+std::vector<Span> plan_levels(const std::vector<int32_t> &counts, int lo, int hi) {
+  std::vector<Span> out;
+  int L = lo;
+  while (L <= hi) {
+    if (counts[L] > kSmallLevel) {
+      out.push_back({L, L, false});
+      L++;
+    } else {
+      int e = L;
+      while (e + 1 <= hi && counts[e + 1] <= kSmallLevel) e++;
+      out.push_back({L, e, true});
+      L = e + 1;
+    }
+  }
+  return out;
+}
+
+template <int NMODE>
+int launch_erosion(fs_model *m, const std::vector<Span> &spans) {
+  int32_t *hdr = m->hdr;
+  for (const Span &s : spans) {
+    if (s.small) {
+      FS_TRY(kt_begin(m, FS_K_EROSION));
+      hipLaunchKernelGGL(k_erode_small<NMODE>, dim3(1), dim3(kSmallBlock), 0, m->stream, m->stack,
+                         m->rcv, m->cnt, m->powtab, m->h, m->levels_d, s.lo, s.hi, m->W, m->N,
+                         m->ek, hdr);
+      FS_TRY(check_launch());
+      FS_TRY(kt_end(m));
+    } else {
+      const int32_t ls = m->levels[s.lo], le = m->levels[s.lo + 1];
+      const unsigned blocks = static_cast<unsigned>((le - ls + 255) / 256);
+      FS_TRY(kt_begin(m, FS_K_EROSION));
+      hipLaunchKernelGGL(k_erode<NMODE>, dim3(blocks), dim3(256), 0, m->stream, m->stack, m->rcv,
+                         m->cnt, m->powtab, m->h, ls, le, m->W, m->N, m->ek, hdr);
+      FS_TRY(check_launch());
+      FS_TRY(kt_end(m));
+    }
+  }
+  return FS_OK;
+}
+
+// One time-step.
+int step_once(fs_model *m) {
+  hipStream_t s = m->stream;
+  hipEvent_t *ev = m->ev[m->ev_cur];
+  const int W = m->W, H = m->H;
+  const int32_t N = m->N;
+
+  FS_HIP_TRY(hipEventRecord(ev[0], s));
+  {
+    const unsigned gx = static_cast<unsigned>((W - 4 + 63) / 64);
+    const unsigned gy = static_cast<unsigned>(std::min((H - 4 + 3) / 4, 65535));
+    FS_TRY(kt_begin(m, FS_K_RECEIVERS));
+    hipLaunchKernelGGL(k_receivers, dim3(gx, gy), dim3(256), 0, s, m->h, m->rcv, W, H,
+                       m->d_card, m->d_diag);
+    FS_TRY(check_launch());
+    FS_TRY(kt_end(m));
+  }
+  FS_HIP_TRY(hipEventRecord(ev[1], s));
+  {
+    const int nclear = m->prev_maxdepth + 1;
+    hipLaunchKernelGGL(k_fill_counts, dim3((nclear + 255) / 256), dim3(256), 0, s, m->hdr, nclear);
+    FS_TRY(check_launch());
+    FS_TRY(kt_begin(m, FS_K_TOPOLOGY));
+    hipLaunchKernelGGL(k_topology, dim3(std::min(H, kTopoBlocks)), dim3(256), 0, s, m->rcv,
+                       m->dmask, m->depth, m->jump, m->list, m->hdr, W, H);
+    FS_TRY(check_launch());
+    FS_TRY(kt_end(m));
+  }
+  FS_HIP_TRY(hipEventRecord(ev[2], s));
+  FS_TRY(fetch_header(m, kHdrCounts));
+  FS_TRY(stage_collect(m, m->ev_cur ^ 1));
+  FS_TRY(kt_collect(m));
+  FS_TRY(device_error_status(m->hdr_h[2]));
+  if (m->hdr_h[1] > 0) {
+    for (int pass = 0; pass < kJumpPasses; pass++) {
+      FS_TRY(kt_begin(m, FS_K_JUMP));
+      hipLaunchKernelGGL(k_jump, dim3(kJumpBlocks), dim3(256), 0, s, m->list, m->depth, m->jump,
+                         m->hdr);
+      FS_TRY(check_launch());
+      FS_TRY(kt_end(m));
+    }
+    FS_TRY(fetch_header(m, kHdrCounts));
+    FS_TRY(device_error_status(m->hdr_h[2]));
+  }
+  const int D = m->hdr_h[0];
+  if (D < 0 || D > FS_MAX_LEVELS) return FS_ERR_LEVEL_OVERFLOW;
+  if (D + 1 > kHdrCounts) FS_TRY(fetch_header(m, D + 1));
+  m->prev_maxdepth = D;
+  m->counts.assign(m->hdr_h + kHdrInts, m->hdr_h + kHdrInts + D + 1);
+  long long total = 0;
+  for (int32_t c : m->counts) total += c;
+  if (total != N) {
+    fs::set_last_error("level histogram covers " + std::to_string(total) + " of " +
+                       std::to_string(N) + " cells");
+    return m->hdr_h[1] > 0 ? FS_ERR_LEVEL_OVERFLOW : FS_ERR_INTERNAL;
+  }
+  // levels[] exactly as GenerateOrder leaves it (RB:218-251): offsets of
+  // levels 0..D, the stack size, and the trailing duplicate; nlevel = D+3.
+  m->levels.resize(D + 3);
+  m->levels[0] = 0;
+  for (int L = 0; L <= D; L++) m->levels[L + 1] = m->levels[L] + m->counts[L];
+  m->levels[D + 2] = N;
+  m->nlevel = D + 3;
+  std::memcpy(m->levels_h, m->levels.data(), m->levels.size() * sizeof(int32_t));
+  FS_HIP_TRY(hipMemcpyAsync(m->levels_d, m->levels_h, m->levels.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice, s));
+
+  // ---- GenerateOrder ----
+  {
+    const unsigned tiles = static_cast<unsigned>((N + kRootTile - 1) / kRootTile);
+    FS_TRY(kt_begin(m, FS_K_ROOTS));
+    hipLaunchKernelGGL(k_roots, dim3(tiles), dim3(kScanBlock), 0, s, m->rcv, m->stack,
+                       m->status, m->ticket, m->ticket_base, next_epoch(m), N, m->hdr);
+    FS_TRY(check_launch());
+    FS_TRY(kt_end(m));
+    m->ticket_base += tiles;
+  }
+  if (D >= 1) {
+    for (const Span &sp : plan_levels(m->counts, 0, D - 1)) {
+      if (sp.small) {
+        FS_TRY(kt_begin(m, FS_K_BFS));
+        hipLaunchKernelGGL(k_bfs_small, dim3(1), dim3(kSmallBlock), 0, s, m->stack, m->dmask,
+                           m->levels_d, sp.lo, sp.hi + 1, W, N, m->hdr);
+        FS_TRY(check_launch());
+        FS_TRY(kt_end(m));
+      } else {
+        const int32_t ls = m->levels[sp.lo], le = m->levels[sp.lo + 1];
+        const unsigned tiles = static_cast<unsigned>((le - ls + kBfsTile - 1) / kBfsTile);
+        FS_TRY(kt_begin(m, FS_K_BFS));
+        hipLaunchKernelGGL(k_bfs, dim3(tiles), dim3(kScanBlock), 0, s, m->stack, m->dmask, ls, le,
+                           W, N, m->status, m->ticket, m->ticket_base, next_epoch(m), m->hdr);
+        FS_TRY(check_launch());
+        FS_TRY(kt_end(m));
+        m->ticket_base += tiles;
+      }
+    }
+  }
+  FS_HIP_TRY(hipEventRecord(ev[3], s));
+
+  // ---- FlowAcc: levels D down to 0 ----
+  {
+    std::vector<Span> spans = plan_levels(m->counts, 0, D);
+    for (auto it = spans.rbegin(); it != spans.rend(); ++it) {
+      if (it->small) {
+        FS_TRY(kt_begin(m, FS_K_FLOWACC));
+        hipLaunchKernelGGL(k_flowacc_small, dim3(1), dim3(kSmallBlock), 0, s, m->stack, m->dmask,
+                           m->cnt, m->levels_d, it->hi, it->lo, W, N, m->hdr);
+        FS_TRY(check_launch());
+        FS_TRY(kt_end(m));
+      } else {
+        const int32_t ls = m->levels[it->lo], le = m->levels[it->lo + 1];
+        FS_TRY(kt_begin(m, FS_K_FLOWACC));
+        hipLaunchKernelGGL(k_flowacc, dim3(static_cast<unsigned>((le - ls + 255) / 256)), dim3(256),
+                           0, s, m->stack, m->dmask, m->cnt, ls, le, W, N, m->hdr);
+        FS_TRY(check_launch());
+        FS_TRY(kt_end(m));
+      }
+    }
+  }
+  FS_HIP_TRY(hipEventRecord(ev[4], s));
+
+  // ---- Uplift of the interior outlets ----
+  {
+    const int32_t n0 = m->counts[0];
+    FS_TRY(kt_begin(m, FS_K_UPLIFT));
+    hipLaunchKernelGGL(k_uplift_roots, dim3(static_cast<unsigned>((n0 + 255) / 256)), dim3(256), 0,
+                       s, m->stack, n0, m->h, W, H, m->ek.ud, N, m->hdr);
+    FS_TRY(check_launch());
+    FS_TRY(kt_end(m));
+  }
+  FS_HIP_TRY(hipEventRecord(ev[5], s));
+
+  // ---- Erode: levels 1..D ----
+  if (D >= 1) {
+    const std::vector<Span> spans = plan_levels(m->counts, 1, D);
+    if (m->nmode == 1) FS_TRY(launch_erosion<1>(m, spans));
+    else if (m->nmode == 2) FS_TRY(launch_erosion<2>(m, spans));
+    else FS_TRY(launch_erosion<0>(m, spans));
+  }
+  FS_HIP_TRY(hipEventRecord(ev[6], s));
+  m->ev_pending[m->ev_cur] = true;
+  m->ev_cur ^= 1;
+  m->have_step = true;
+  return FS_OK;
+}
+
+int finish(fs_model *m) {
+  FS_HIP_TRY(hipStreamSynchronize(m->stream));
+  FS_TRY(stage_collect(m, 0));
+  FS_TRY(stage_collect(m, 1));
+  FS_TRY(kt_collect(m));
+  FS_TRY(fetch_header(m, 0));
+  return device_error_status(m->hdr_h[2]);
+}
+
+int run_steps(fs_model *m, long long iterations) {
+  if (!m) return FS_ERR_INVALID_ARGUMENT;
+  if (iterations < 0) return FS_ERR_INVALID_ARGUMENT;
+  FS_HIP_TRY(hipSetDevice(m->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  // clear the device error word left by a previous call
+  FS_HIP_TRY(hipMemcpyAsync(m->hdr_h, m->hdr, kHdrInts * sizeof(int32_t), hipMemcpyDeviceToHost,
+                            m->stream));
+  FS_HIP_TRY(hipStreamSynchronize(m->stream));
+  if (m->hdr_h[2] != 0) {
+    const int32_t zero = 0;
+    FS_HIP_TRY(hipMemcpy(m->hdr + 2, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  }
+  int st = FS_OK;
+  for (long long i = 0; i < iterations && st == FS_OK; i++) st = step_once(m);
+  const int fst = finish(m);
+  if (st == FS_OK) st = fst;
+  for (int i = 0; i < 6; i++) {
+    const double ms = m->stage_ms[FS_T_RECEIVERS + i];
+    const uint64_t whole = static_cast<uint64_t>(ms * 1000.0);
+    m->timer_us[FS_T_RECEIVERS + i] += whole;
+    m->stage_ms[FS_T_RECEIVERS + i] = ms - static_cast<double>(whole) / 1000.0;
+  }
+  m->timer_us[FS_T_OVERALL] += static_cast<uint64_t>(
+      std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t0)
+          .count());
+  return st;
+}
+
+int check_model(fs_model *m) {
+  if (!m) return FS_ERR_INVALID_ARGUMENT;
+  const hipError_t e = hipSetDevice(m->device);
+  if (e != hipSuccess) {
+    fs::set_last_error(std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return FS_ERR_HIP;
+  }
+  return FS_OK;
+}
+
+template <class T>
+int download(fs_model *m, T *dst, const T *src, size_t n) {
+  FS_HIP_TRY(hipStreamSynchronize(m->stream));
+  FS_HIP_TRY(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyDeviceToHost));
+  return FS_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int fs_params_preset(int preset, fs_params *out) {
+  if (!out) return FS_ERR_INVALID_ARGUMENT;
+  if (preset == FS_PRESET_REFERENCE) {       // RB:58-65
+    *out = fs_params{2e-6, 2, 0.8, 2e-3, 1000., 1., 40000, 1e-3};
+    return FS_OK;
+  }
+  if (preset == FS_PRESET_BENCHMARK) {
+    *out = fs_params{1e-5, 1, 0.4, 2e-3, 1000., 100., 1e4, 1e-3};
+    return FS_OK;
+  }
+  return FS_ERR_INVALID_ARGUMENT;
+}
+
+int fs_create(int width, int height, const fs_params *params, uint32_t engine_seed, int device,
+              fs_model **out) {
+  return create_model(width, height, params, nullptr, engine_seed, device, out);
+}
+
+int fs_create_from_h(int width, int height, const fs_params *params, const double *h, int device,
+                     fs_model **out) {
+  if (!h) return FS_ERR_INVALID_ARGUMENT;
+  return create_model(width, height, params, h, 0, device, out);
+}
+
+int fs_destroy(fs_model *m) {
+  if (!m) return FS_ERR_INVALID_ARGUMENT;
+  free_model(m);
+  return FS_OK;
+}
+
+int fs_run(fs_model *m, int nstep) {
+  if (nstep < 0) return FS_ERR_INVALID_ARGUMENT;
+  return run_steps(m, static_cast<long long>(nstep) + 1);   // RB:366
+}
+
+int fs_step(fs_model *m, int iterations) {
+  if (iterations < 0) return FS_ERR_INVALID_ARGUMENT;
+  return run_steps(m, iterations);
+}
+
+int fs_get_h(fs_model *m, double *out, size_t count) {
+  FS_TRY(check_model(m));
+  if (!out) return FS_ERR_INVALID_ARGUMENT;
+  if (count != static_cast<size_t>(m->N)) return FS_ERR_BUFFER_SIZE;
+  return download(m, out, m->h, count);
+}
+
+int fs_set_h(fs_model *m, const double *in, size_t count) {
+  FS_TRY(check_model(m));
+  if (!in) return FS_ERR_INVALID_ARGUMENT;
+  if (count != static_cast<size_t>(m->N)) return FS_ERR_BUFFER_SIZE;
+  FS_HIP_TRY(hipStreamSynchronize(m->stream));
+  FS_HIP_TRY(hipMemcpy(m->h, in, count * sizeof(double), hipMemcpyHostToDevice));
+  return FS_OK;
+}
+
+int fs_get_timers(fs_model *m, uint64_t us[FS_NUM_TIMERS]) {
+  if (!m || !us) return FS_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < FS_NUM_TIMERS; i++) us[i] = m->timer_us[i];
+  return FS_OK;
+}
+
+int fs_reset_timers(fs_model *m) {
+  if (!m) return FS_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < FS_NUM_TIMERS; i++) {
+    m->timer_us[i] = 0;
+    m->stage_ms[i] = 0;
+  }
+  return FS_OK;
+}
+
+int fs_get_receivers(fs_model *m, int32_t *rec, size_t count) {
+  FS_TRY(check_model(m));
+  if (!rec) return FS_ERR_INVALID_ARGUMENT;
+  if (count != static_cast<size_t>(m->N)) return FS_ERR_BUFFER_SIZE;
+  std::vector<uint8_t> r(count);
+  FS_TRY(download(m, r.data(), m->rcv, count));
+  for (size_t i = 0; i < count; i++) rec[i] = r[i] == kNoFlow ? FS_NO_FLOW : r[i];
+  return FS_OK;
+}
+
+int fs_get_donors(fs_model *m, int32_t *ndon, int32_t *donor, size_t count) {
+  FS_TRY(check_model(m));
+  if (!ndon || !donor) return FS_ERR_INVALID_ARGUMENT;
+  if (count != static_cast<size_t>(m->N)) return FS_ERR_BUFFER_SIZE;
+  std::vector<uint8_t> mk(count);
+  FS_TRY(download(m, mk.data(), m->dmask, count));
+  const int W = m->W;
+  const int off[8] = {-W - 1, -W, -W + 1, -1, 1, W - 1, W, W + 1};
+  for (size_t c = 0; c < count; c++) {
+    int k = 0;
+    for (int j = 0; j < 8; j++)
+      if (mk[c] & (1u << j)) donor[8 * c + k++] = static_cast<int32_t>(c) + off[j];
+    ndon[c] = k;
+    for (; k < 8; k++) donor[8 * c + k] = 0;
+  }
+  return FS_OK;
+}
+
+int fs_get_stack(fs_model *m, int32_t *stack, size_t count) {
+  FS_TRY(check_model(m));
+  if (!stack) return FS_ERR_INVALID_ARGUMENT;
+  if (count != static_cast<size_t>(m->N)) return FS_ERR_BUFFER_SIZE;
+  return download(m, stack, m->stack, count);
+}
+
+int fs_get_levels(fs_model *m, int32_t *levels, size_t capacity, int32_t *nlevel) {
+  if (!m || !nlevel) return FS_ERR_INVALID_ARGUMENT;
+  *nlevel = m->have_step ? m->nlevel : 0;
+  if (!levels) return FS_OK;
+  if (capacity < static_cast<size_t>(*nlevel)) return FS_ERR_BUFFER_SIZE;
+  std::copy(m->levels.begin(), m->levels.begin() + *nlevel, levels);
+  return FS_OK;
+}
+
+int fs_get_accum(fs_model *m, double *accum, size_t count) {
+  FS_TRY(check_model(m));
+  if (!accum) return FS_ERR_INVALID_ARGUMENT;
+  if (count != static_cast<size_t>(m->N)) return FS_ERR_BUFFER_SIZE;
+  std::vector<uint32_t> c(count);
+  FS_TRY(download(m, c.data(), m->cnt, count));
+  for (size_t i = 0; i < count; i++) accum[i] = static_cast<double>(c[i]) * m->p.cell_area;
+  return FS_OK;
+}
+
+int fs_device_h(fs_model *m, double **dptr) {
+  if (!m || !dptr) return FS_ERR_INVALID_ARGUMENT;
+  *dptr = m->h;
+  return FS_OK;
+}
+
+int fs_stream(fs_model *m, void **stream) {
+  if (!m || !stream) return FS_ERR_INVALID_ARGUMENT;
+  *stream = static_cast<void *>(m->stream);
+  return FS_OK;
+}
+
+int fs_synchronize(fs_model *m) {
+  FS_TRY(check_model(m));
+  FS_HIP_TRY(hipStreamSynchronize(m->stream));
+  return FS_OK;
+}
+
+int fs_set_kernel_timing(fs_model *m, int enable) {
+  FS_TRY(check_model(m));
+  FS_HIP_TRY(hipStreamSynchronize(m->stream));
+  FS_TRY(kt_collect(m));
+  m->ktiming = enable != 0;
+  for (int i = 0; i < FS_NUM_KERNELS; i++) {
+    m->kms[i] = 0;
+    m->klaunches[i] = 0;
+  }
+  return FS_OK;
+}
+
+int fs_kernel_stats(fs_model *m, int kernel, double *total_ms, uint64_t *launches) {
+  if (!m || kernel < 0 || kernel >= FS_NUM_KERNELS) return FS_ERR_INVALID_ARGUMENT;
+  if (total_ms) *total_ms = m->kms[kernel];
+  if (launches) *launches = m->klaunches[kernel];
+  return FS_OK;
+}
+
+const char *fs_kernel_name(int kernel) {
+  static const char *names[FS_NUM_KERNELS] = {"k_receivers", "k_topology", "k_roots", "k_bfs",
+                                              "k_flowacc",   "k_uplift_roots", "k_erode",
+                                              "k_jump"};
+  if (kernel < 0 || kernel >= FS_NUM_KERNELS) return "unknown";
+  return names[kernel];
+}
+
+}  // extern "C"

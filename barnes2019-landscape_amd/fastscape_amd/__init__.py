@@ -1,0 +1,13 @@
+"""MI355X FastScape time-step: Python host binding of libfastscape_hip.so.
+
+The compute path is the HIP library (csrc/); this package only loads it and
+mirrors the reference class surface (fastscape_RB.cpp:46-406) for tests,
+benchmarks and scripting.
+"""
+from ._lib import (EXPORTED, FS_MAX_LEVELS, FS_NO_FLOW, KERNELS, LIB_PATH, PRESET_BENCHMARK,
+                   PRESET_REFERENCE, TIMER_NAMES, FastScape, FastScapeError, FsParams,
+                   generate_terrain, library, preset, write_dem)
+
+__all__ = ["EXPORTED", "FS_MAX_LEVELS", "FS_NO_FLOW", "KERNELS", "LIB_PATH",
+           "PRESET_BENCHMARK", "PRESET_REFERENCE", "TIMER_NAMES", "FastScape",
+           "FastScapeError", "FsParams", "generate_terrain", "library", "preset", "write_dem"]

@@ -1,0 +1,141 @@
+#!/usr/bin/env python3
+"""Benchmark: FastScape time-steps on one MI355X.
+
+A step is one pass of the time-step (receivers, donors, order, flow
+accumulation, uplift, erosion: fastscape_RB.cpp:366-376) over the whole grid.
+Default workload: 10000 x 10000 cells, FP64, benchmark preset (K=1e-5, n=1,
+m=0.4, U=2e-3, dt=1000, dx=dy=100), terrain from mt19937 seeded 0 (the
+reference's terrain for any nonzero seed), state resident in HBM before the
+timed region.
+
+    python bench.py [--gpus 1] [--steps 121] [--warmup 5] [--size 10000]
+
+Prints one JSON line.  `value` is cell-steps/s over the timed steps;
+`roofline` is the dominant kernel's algorithmic bytes / its HIP-event time;
+`cpu_baseline` times the reference's RB+P (OpenMP) on a bounded sample on
+this host.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "barnes2019-landscape_amd"))
+
+METRIC = "cell-steps/s at 10000² FP64 vs RB+P CPU; exact-vs-proposal inversion wall time"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per processed cell for each kernel (reads + writes
+# of the minimal data the kernel must touch; neighbour re-reads excluded).
+KERNEL_BYTES = {
+    "k_receivers": 9,      # h 8 + rcv 1
+    "k_topology": 4,       # rcv 1 + donor mask 1 + level 2
+    "k_bfs": 9,            # stack entry 4 + donor mask 1 + donor index out 4
+    "k_flowacc": 13,       # stack 4 + mask 1 + donor counts 4 (one donor/cell) + count 4
+    "k_erode": 33,         # stack 4 + rcv 1 + count 4 + h 8 + receiver h 8 + h out 8
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=121)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=10000)
+    ap.add_argument("--profile-steps", type=int, default=10,
+                    help="steps timed per kernel with HIP events for the roofline")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-size", type=int, default=10000)
+    ap.add_argument("--cpu-sample-run", type=int, default=1,
+                    help="run(n) of the CPU sample, i.e. n+1 time-steps")
+    return ap.parse_args()
+
+
+def cpu_baseline(size, nrun):
+    """Reference RB+P (fastscape_RB+P.cpp, OpenMP) built from the reference
+    sources under oracle/_ref; falls back to the oracle's own RB+P
+    restatement when the reference build is absent."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_rbp")
+    if os.path.exists(ref):
+        cmd = [ref, "run", str(size), str(size), str(nrun), "123", "1", "-"]
+        kind = "reference"
+    else:
+        cmd = [os.path.join(ROOT, "oracle", "oracle_cli"), str(size), str(size), str(nrun), "0",
+               "1", "0", "1", "-"]
+        kind = "port"
+    out = subprocess.run(cmd, check=True, capture_output=True, text=True, env=env).stdout
+    secs = float(re.search(r"m run_seconds = ([0-9.eE+-]+)", out).group(1))
+    cells = size * size * (nrun + 1)
+    return {"value": cells / secs, "unit": "cell-steps/s", "cores": threads, "kind": kind,
+            "sample": "%dx%d benchmark preset, run(%d) = %d time-steps, %.1f s, %s" % (
+                size, size, nrun, nrun + 1, secs,
+                "fastscape_RB+P.cpp" if kind == "reference" else "oracle RB+P restatement")}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(independent realizations shard by whole grids, see DESIGN.md)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import fastscape_amd as fa
+
+    W = H = args.size
+    N = W * H
+    m = fa.FastScape(W, H, fa.preset(fa.PRESET_BENCHMARK), engine_seed=0, device=0)
+    m.step(args.warmup)
+    m.synchronize()
+    t0 = time.perf_counter()
+    m.step(args.steps)
+    m.synchronize()
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    value = N * args.steps / secs
+
+    # per-kernel HIP-event timing over a further window of steps
+    m.set_kernel_timing(True)
+    m.step(args.profile_steps)
+    stats = m.kernel_stats()
+    m.set_kernel_timing(False)
+    levels = m.levels()
+    n0 = int(levels[1]) if levels.size > 1 else N
+    units = {"k_receivers": (W - 4) * (H - 4), "k_topology": N, "k_bfs": N - n0,
+             "k_flowacc": N, "k_erode": N - n0}
+    dom = max(stats, key=lambda k: stats[k][0])
+    ms, launches = stats[dom]
+    per_step_bytes = KERNEL_BYTES.get(dom, 0) * units.get(dom, N)
+    achieved = per_step_bytes * args.profile_steps / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": dom, "launches_per_step": launches / args.profile_steps,
+                "kernel_ms_per_step": ms / args.profile_steps}
+    m.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_sample_size, args.cpu_sample_run)
+
+    line = {"metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": secs * 1e3 / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (mt19937 seed 0 uniform[0,1) terrain, rings 0/1 at 0)",
+            "config": {"workload": "FastScape %dx%d FP64 time-steps, K=1e-5 n=1 m=0.4 U=2e-3 "
+                                   "dt=1000 dx=100" % (W, H),
+                       "grid": [W, H], "nlevel_last_step": int(levels.size)},
+            "roofline": roofline, "cpu_baseline": cpu,
+            "stage_kernel_ms_per_step": {k: round(v[0] / args.profile_steps, 4)
+                                         for k, v in stats.items()}}
+    if cpu:
+        line["speedup_vs_cpu_baseline"] = value / cpu["value"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,187 @@
+/*
+ * fastscape.h -- C ABI of the MI355X FastScape time-step library
+ * (libfastscape_hip.so).
+ *
+ * The reference keeps the whole model in one C++ class per variant
+ * (class FastScape_RB, fastscape_RB.cpp:46-406).  This header is the
+ * language-neutral boundary that replaces it: plain pointers and sizes, no
+ * C++ or torch types, status codes instead of asserts and exceptions.  The
+ * C++ drop-in class with the reference's own surface is fastscape.hpp.
+ *
+ * Grid conventions are the reference's: row-major, cell c = y*W + x, ring 0
+ * (the outer edge) is a halo, ring 1 is held at elevation 0, interior cells
+ * are [2, W-2) x [2, H-2) (fastscape_RB.cpp:73-77, 118-128).
+ *
+ * Threading: one host thread per model at a time; models are independent.
+ * Every model owns one HIP stream; all device work of a model is ordered
+ * on it.
+ */
+#ifndef FASTSCAPE_H
+#define FASTSCAPE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------
+ * Replace assert(levels[nlevel-1]==nstack) (fastscape_RB.cpp:253), the
+ * timer's runtime_error (CumulativeTimer.cpp:21) and the usage exit
+ * (fastscape_RB.cpp:418-421). */
+#define FS_OK                    0
+#define FS_ERR_INVALID_ARGUMENT  1  /* null pointer, bad preset, nstep < 0  */
+#define FS_ERR_DIMENSIONS        2  /* W or H < 5, or W*H >= 2^31           */
+#define FS_ERR_OUT_OF_MEMORY     3  /* host or device allocation failed     */
+#define FS_ERR_HIP               4  /* a HIP runtime call failed            */
+#define FS_ERR_LEVEL_OVERFLOW    5  /* drainage deeper than FS_MAX_LEVELS   */
+#define FS_ERR_NEWTON            6  /* a Newton solve did not converge      */
+#define FS_ERR_NO_DEVICE         7  /* no HIP device / bad device ordinal   */
+#define FS_ERR_BUFFER_SIZE       8  /* caller buffer has the wrong size     */
+#define FS_ERR_INTERNAL          9  /* an internal consistency check failed */
+#define FS_ERR_IO               10  /* file could not be written            */
+
+/* Direction code of a cell without a receiver (fastscape_RB.cpp:50). */
+#define FS_NO_FLOW (-1)
+/* Deepest drainage supported: a cell may be at most this many receiver hops
+ * from its outlet (the reference sizes its level buffer 2W+2H,
+ * fastscape_RB.cpp:358, and overruns it silently beyond that). */
+#define FS_MAX_LEVELS 65533
+
+/* ---- parameters -----------------------------------------------------------
+ * The reference's constants are const class members (fastscape_RB.cpp:58-65);
+ * here they are a parameter block.  Receiver distances are
+ * dr = {dx, dx*SQRT2} for cardinal / diagonal neighbours
+ * (fastscape_RB.cpp:63, with dx = 1 there). */
+typedef struct fs_params {
+  double keq;        /* K   stream-power coefficient      RB:58 */
+  double neq;        /* n   slope exponent                RB:59 */
+  double meq;        /* m   drainage-area exponent        RB:60 */
+  double ueq;        /* U   uplift rate                   RB:61 */
+  double dt;         /*     time step                     RB:62 */
+  double dx;         /*     cell spacing                  RB:63 */
+  double cell_area;  /*     area of one cell              RB:65 */
+  double tol;        /*     Newton tolerance              RB:64 */
+} fs_params;
+
+#define FS_PRESET_REFERENCE 0  /* the reference's constants (RB:58-65)      */
+#define FS_PRESET_BENCHMARK 1  /* K=1e-5 n=1 m=0.4 U=2e-3 dt=1000 dx=100
+                                  area=1e4                                  */
+
+/* Timer slots, in the order of the reference's timers (RB:100-107). */
+#define FS_T_INITIALIZE 0
+#define FS_T_RECEIVERS  1
+#define FS_T_DONORS     2
+#define FS_T_ORDER      3
+#define FS_T_FLOWACC    4
+#define FS_T_UPLIFT     5
+#define FS_T_EROSION    6
+#define FS_T_OVERALL    7
+#define FS_NUM_TIMERS   8
+
+typedef struct fs_model fs_model;
+
+/* Fill *out with a preset.  No counterpart: the reference hard-codes the
+ * constants (RB:58-65). */
+int fs_params_preset(int preset, fs_params *out);
+
+/* Construct a model: allocate device state on `device` and fill the terrain
+ * with GenerateRandomTerrain's recipe (RB:111-130): one uniform[0,1) double
+ * per cell in row-major order from mt19937(engine_seed) via libstdc++'s
+ * generate_canonical (random.cpp:45-49), rings 0 and 1 set to 0.
+ * Replaces FastScape_RB::FastScape_RB(width, height) (RB:135-151).  The
+ * reference seeds its engine with seed*thread (random.cpp:31), so a nonzero
+ * seed_rand() argument means engine_seed 0; fastscape.hpp keeps that mapping. */
+int fs_create(int width, int height, const fs_params *params,
+              uint32_t engine_seed, int device, fs_model **out);
+
+/* Same, with the caller's initial elevations (W*H doubles, row-major). */
+int fs_create_from_h(int width, int height, const fs_params *params,
+                     const double *h, int device, fs_model **out);
+
+/* Free all host and device memory of the model.  Replaces ~FastScape_RBGPU
+ * (fastscape_RB+GPU.cpp:123-125). */
+int fs_destroy(fs_model *m);
+
+/* run(nstep): nstep+1 time-steps, as the reference's
+ * `for(step=0; step<=nstep; step++)` (RB:341-398).  Resumable: the state is
+ * the elevation field, so consecutive calls continue the same run.
+ * Timers accumulate.  Prints nothing (the C++ wrapper prints the reference's
+ * progress and timer lines). */
+int fs_run(fs_model *m, int nstep);
+
+/* Exactly `iterations` time-steps (no +1).  No reference counterpart; used
+ * by benchmarks and per-step parity checks. */
+int fs_step(fs_model *m, int iterations);
+
+/* Copy the elevations (W*H doubles) device -> host.  Replaces getH()
+ * (RB:403-405), which returns the reference's internal vector. */
+int fs_get_h(fs_model *m, double *out, size_t count);
+
+/* Overwrite the elevations (W*H doubles) host -> device (resume / edit, the
+ * use RB:334-340 describes). */
+int fs_set_h(fs_model *m, const double *in, size_t count);
+
+/* Per-stage times in microseconds, in FS_T_* order (RB:380-387).  Stage
+ * times are device time between HIP events bracketing each stage; Overall is
+ * host wall time of fs_run / fs_step plus construction. */
+int fs_get_timers(fs_model *m, uint64_t us[FS_NUM_TIMERS]);
+int fs_reset_timers(fs_model *m);
+
+/* ---- state of the last completed time-step (parity checks) -------------- */
+/* rec[] in the reference encoding: 0..7 or FS_NO_FLOW (RB:86, 160-182). */
+int fs_get_receivers(fs_model *m, int32_t *rec, size_t count);
+/* ndon[count] and donor[8*count] as ComputeDonors leaves them
+ * (RB:87-88, 188-203): donor lists in ascending donor index; slots beyond
+ * ndon are written as 0. */
+int fs_get_donors(fs_model *m, int32_t *ndon, int32_t *donor, size_t count);
+/* stack[] (RB:89, 213-254): the full level order, `count` = W*H. */
+int fs_get_stack(fs_model *m, int32_t *stack, size_t count);
+/* levels[] (RB:96) and nlevel (RB:97).  `*nlevel` receives the count; pass
+ * levels = NULL to query it. */
+int fs_get_levels(fs_model *m, int32_t *levels, size_t capacity, int32_t *nlevel);
+/* accum[] (RB:85, 262-279), cell_area times the upstream cell count. */
+int fs_get_accum(fs_model *m, double *accum, size_t count);
+
+/* ---- device access --------------------------------------------------------
+ * Device pointer of the elevation field (W*H doubles on the model's device)
+ * and the model's stream (a hipStream_t).  Work a caller enqueues on other
+ * streams must be ordered against this stream by the caller. */
+int fs_device_h(fs_model *m, double **dptr);
+int fs_stream(fs_model *m, void **stream);
+int fs_synchronize(fs_model *m);
+
+/* ---- kernel timing ---------------------------------------------------------
+ * When enabled, every launch of each kernel is bracketed by HIP events on the
+ * model's stream and its duration accumulated per kernel.  Kernel ids:
+ * FS_K_* below.  fs_kernel_stats returns total milliseconds and launches. */
+#define FS_K_RECEIVERS 0
+#define FS_K_TOPOLOGY  1
+#define FS_K_ROOTS     2
+#define FS_K_BFS       3
+#define FS_K_FLOWACC   4
+#define FS_K_UPLIFT    5
+#define FS_K_EROSION   6
+#define FS_K_JUMP      7
+#define FS_NUM_KERNELS 8
+int fs_set_kernel_timing(fs_model *m, int enable);
+int fs_kernel_stats(fs_model *m, int kernel, double *total_ms, uint64_t *launches);
+const char *fs_kernel_name(int kernel);
+
+/* ---- host helpers ---------------------------------------------------------- */
+/* PrintDEM (RB:20-40): ArcGIS ASCII grid of the inner (W-2)x(H-2) cells,
+ * byte-identical to the reference writer. */
+int fs_write_dem(const char *filename, const double *h, int width, int height);
+/* GenerateRandomTerrain's recipe on the host (RB:111-130). */
+int fs_generate_terrain(double *h, int width, int height, uint32_t engine_seed);
+/* Message for a status code; never NULL. */
+const char *fs_strerror(int status);
+/* Detail of the last error raised on this thread (e.g. the failing HIP
+ * call); empty string if none. */
+const char *fs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FASTSCAPE_H */
