@@ -530,6 +530,14 @@ __global__ __launch_bounds__(256) void k_uplift_roots(const int32_t *__restrict_
   if (x >= 2 && x < W - 2 && y >= 2 && y < H - 2) h[c] += ud;
 }
 
+// Plain 16-byte-per-thread copy, used only to calibrate hardware traffic
+// counters against a kernel whose HBM bytes are known exactly.
+__global__ __launch_bounds__(256) void k_copy_calibration(const uint4 *__restrict__ src,
+                                                          uint4 *__restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += gridDim.x * 256ull)
+    dst[i] = src[i];
+}
+
 struct ErodeConst {
   double kdt;       // keq*dt
   double lenpow0;   // pow(dx, neq)        cardinal receiver
@@ -1327,6 +1335,39 @@ int fs_kernel_stats(fs_model *m, int kernel, double *total_ms, uint64_t *launche
   if (total_ms) *total_ms = m->kms[kernel];
   if (launches) *launches = m->klaunches[kernel];
   return FS_OK;
+}
+
+int fs_calibrate_copy(size_t nbytes, int device, double *ms) {
+  if (!ms || nbytes < 16) return FS_ERR_INVALID_ARGUMENT;
+  FS_HIP_TRY(hipSetDevice(device));
+  const size_t n = nbytes / 16;
+  uint4 *a = nullptr, *b = nullptr;
+  FS_TRY(dev_alloc(&a, n));
+  if (dev_alloc(&b, n) != FS_OK) {
+    (void)hipFree(a);
+    return FS_ERR_OUT_OF_MEMORY;
+  }
+  hipEvent_t e0, e1;
+  int st = FS_OK;
+  if (hipMemset(a, 1, n * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    st = FS_ERR_HIP;
+  } else {
+    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 1u << 20));
+    hipLaunchKernelGGL(k_copy_calibration, dim3(blocks), dim3(256), 0, nullptr, a, b, n);
+    (void)hipEventRecord(e0, nullptr);
+    hipLaunchKernelGGL(k_copy_calibration, dim3(blocks), dim3(256), 0, nullptr, a, b, n);
+    (void)hipEventRecord(e1, nullptr);
+    float t = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+      st = FS_ERR_HIP;
+    *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return st;
 }
 
 const char *fs_kernel_name(int kernel) {

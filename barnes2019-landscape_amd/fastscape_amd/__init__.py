@@ -6,8 +6,8 @@ benchmarks and scripting.
 """
 from ._lib import (EXPORTED, FS_MAX_LEVELS, FS_NO_FLOW, KERNELS, LIB_PATH, PRESET_BENCHMARK,
                    PRESET_REFERENCE, TIMER_NAMES, FastScape, FastScapeError, FsParams,
-                   generate_terrain, library, preset, write_dem)
+                   calibrate_copy, generate_terrain, library, preset, write_dem)
 
 __all__ = ["EXPORTED", "FS_MAX_LEVELS", "FS_NO_FLOW", "KERNELS", "LIB_PATH",
            "PRESET_BENCHMARK", "PRESET_REFERENCE", "TIMER_NAMES", "FastScape",
-           "FastScapeError", "FsParams", "generate_terrain", "library", "preset", "write_dem"]
+           "FastScapeError", "FsParams", "calibrate_copy", "generate_terrain", "library", "preset", "write_dem"]

@@ -30,7 +30,7 @@ EXPORTED = [
     "fs_get_h", "fs_set_h", "fs_get_timers", "fs_reset_timers", "fs_get_receivers",
     "fs_get_donors", "fs_get_stack", "fs_get_levels", "fs_get_accum", "fs_device_h",
     "fs_stream", "fs_synchronize", "fs_set_kernel_timing", "fs_kernel_stats",
-    "fs_kernel_name", "fs_write_dem", "fs_generate_terrain", "fs_strerror", "fs_last_error",
+    "fs_kernel_name", "fs_calibrate_copy", "fs_write_dem", "fs_generate_terrain", "fs_strerror", "fs_last_error",
 ]
 
 
@@ -91,6 +91,7 @@ def _lib():
         "fs_set_kernel_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
         "fs_kernel_stats": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _u64p]),
         "fs_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
+        "fs_calibrate_copy": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, _dp]),
         "fs_write_dem": (ctypes.c_int, [ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int]),
         "fs_generate_terrain": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32]),
         "fs_strerror": (ctypes.c_char_p, [ctypes.c_int]),
@@ -134,6 +135,14 @@ def generate_terrain(width, height, engine_seed=0):
     _check(_lib().fs_generate_terrain(_ptr(h, ctypes.c_double), width, height, engine_seed),
            "fs_generate_terrain")
     return h
+
+
+def calibrate_copy(nbytes, device=0):
+    """Time a plain device copy of nbytes (calibrates traffic counters)."""
+    ms = ctypes.c_double(0)
+    _check(_lib().fs_calibrate_copy(int(nbytes), int(device), ctypes.byref(ms)),
+           "fs_calibrate_copy")
+    return ms.value
 
 
 def write_dem(filename, h, width, height):

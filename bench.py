@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--cpu-sample-size", type=int, default=10000)
     ap.add_argument("--cpu-sample-run", type=int, default=1,
                     help="run(n) of the CPU sample, i.e. n+1 time-steps")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a 1 GiB plain copy kernel (counter calibration)")
     return ap.parse_args()
 
 
@@ -87,6 +89,10 @@ def main():
         sys.exit(2)
     import fastscape_amd as fa
 
+    if args.calibrate:
+        ms = fa.calibrate_copy(1 << 30)
+        print("calibration copy: 1 GiB in %.3f ms (%.1f GB/s read+write)"
+              % (ms, 2 * (1 << 30) / ms / 1e6), file=sys.stderr)
     W = H = args.size
     N = W * H
     m = fa.FastScape(W, H, fa.preset(fa.PRESET_BENCHMARK), engine_seed=0, device=0)

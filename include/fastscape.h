@@ -169,6 +169,12 @@ int fs_set_kernel_timing(fs_model *m, int enable);
 int fs_kernel_stats(fs_model *m, int kernel, double *total_ms, uint64_t *launches);
 const char *fs_kernel_name(int kernel);
 
+/* Copy `nbytes` (rounded down to 16) between two device buffers with a plain
+ * streaming kernel, twice; *ms receives the second launch's time.  No
+ * reference counterpart: it calibrates hardware traffic counters (each
+ * launch reads and writes exactly nbytes). */
+int fs_calibrate_copy(size_t nbytes, int device, double *ms);
+
 /* ---- host helpers ---------------------------------------------------------- */
 /* PrintDEM (RB:20-40): ArcGIS ASCII grid of the inner (W-2)x(H-2) cells,
  * byte-identical to the reference writer. */
