@@ -200,6 +200,7 @@ __global__ void k_fill_counts(int32_t *hdr, int n_counts) {
   if (i == 0) {
     hdr[0] = 0;   // maxdepth
     hdr[1] = 0;   // n_unfinished
+    hdr[3] = 0;   // tiles whose levels exceed the tile halo
   }
   if (i < n_counts) hdr[kHdrInts + i] = 0;
 }
@@ -547,32 +548,14 @@ struct ErodeConst {
   double tol;
 };
 
-// Erode for one cell (RB:306-328).  NMODE 1: n = 1, pow(x,1) = x and
-// pow(x,0) = 1 exactly as libm returns them; NMODE 2: n = 2 with x*x for
-// pow(x,2); NMODE 0: any n through the device pow.  h0 is the uplifted
-// elevation h + ueq*dt (RB:294), hn the receiver's final elevation (its
-// level was processed earlier).
+// The implicit update of one cell (RB:313-326), operation for operation.
+// NMODE 1: n = 1, where libm returns pow(x,1) = x and pow(x,0) = 1 exactly;
+// NMODE 2: n = 2 with x*x for pow(x,2); NMODE 0: any n through the device
+// pow.  fact = (K*dt)*A^m / dr^n, h0 is the uplifted elevation h + U*dt
+// (RB:294) and hn the receiver's final elevation.
 template <int NMODE>
-__device__ __forceinline__ void erode_cell(int32_t c, const uint8_t *__restrict__ rcv,
-                                           const uint32_t *__restrict__ cnt,
-                                           const double *__restrict__ powtab, double *h,
-                                           int W, int32_t N, const ErodeConst &k,
-                                           int32_t *hdr) {
-  if (static_cast<uint32_t>(c) >= static_cast<uint32_t>(N)) {
-    atomicOr(&hdr[2], kErrIndex);
-    return;
-  }
-  const int r = rcv[c];
-  if (r >= 8) return;
-  const int32_t n = c + dir_offset(r, W);
-  uint32_t a = cnt[c];
-  if (a > static_cast<uint32_t>(N)) {
-    atomicOr(&hdr[2], kErrIndex);
-    a = static_cast<uint32_t>(N);
-  }
-  const double fact = (k.kdt * powtab[a]) / ((r & 1) ? k.lenpow1 : k.lenpow0);
-  const double h0 = h[c] + k.ud;
-  const double hn = h[n];
+__device__ __forceinline__ double newton_update(double h0, double hn, double fact,
+                                                const ErodeConst &k, int32_t *hdr) {
   double hnew = h0, hp = h0, diff = 2.0 * k.tol;
   int it = 0;
   while (fabs(diff) > k.tol) {
@@ -596,7 +579,31 @@ __device__ __forceinline__ void erode_cell(int32_t c, const uint8_t *__restrict_
       break;
     }
   }
-  h[c] = hnew;
+  return hnew;
+}
+
+// Erode for one stack entry (RB:306-328): h[c] from h[c] + U*dt and the
+// receiver's final elevation (its level was processed earlier).
+template <int NMODE>
+__device__ __forceinline__ void erode_cell(int32_t c, const uint8_t *__restrict__ rcv,
+                                           const uint32_t *__restrict__ cnt,
+                                           const double *__restrict__ powtab, double *h,
+                                           int W, int32_t N, const ErodeConst &k,
+                                           int32_t *hdr) {
+  if (static_cast<uint32_t>(c) >= static_cast<uint32_t>(N)) {
+    atomicOr(&hdr[2], kErrIndex);
+    return;
+  }
+  const int r = rcv[c];
+  if (r >= 8) return;
+  const int32_t n = c + dir_offset(r, W);
+  uint32_t a = cnt[c];
+  if (a > static_cast<uint32_t>(N)) {
+    atomicOr(&hdr[2], kErrIndex);
+    a = static_cast<uint32_t>(N);
+  }
+  const double fact = (k.kdt * powtab[a]) / ((r & 1) ? k.lenpow1 : k.lenpow0);
+  h[c] = newton_update<NMODE>(h[c] + k.ud, h[n], fact, k, hdr);
 }
 
 template <int NMODE>
@@ -622,6 +629,256 @@ __global__ __launch_bounds__(kSmallBlock) void k_erode_small(
     for (long long p = ls + threadIdx.x; p < le; p += kSmallBlock)
       erode_cell<NMODE>(stack[p], rcv, cnt, powtab, h, W, N, k, hdr);
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// tile path
+// ---------------------------------------------------------------------------
+// A cell's receiver chain and its whole upstream tree lie within D cells
+// (Chebyshev distance) of it, D being the deepest level of the grid.  So
+// when D <= kHalo, a kTile x kTile tile plus a kHalo-wide halo holds
+// everything the tile's cells depend on, and accumulation and erosion can
+// run level by level inside LDS with one HBM read of the region and one
+// write of the tile.  Every grid cell is a tile cell of exactly one tile.
+constexpr int kTile = 64;
+constexpr int kHalo = 16;
+constexpr int kReg = kTile + 2 * kHalo;       // region side
+constexpr int kRegCells = kReg * kReg;
+constexpr int kAccBlock = 512;
+constexpr int kEroBlock = 1024;
+constexpr uint8_t kOutside = 0xFD;            // region cell beyond the grid
+constexpr uint8_t kEscaped = 0xFF;            // chain leaves the region
+
+// Region-local neighbour step in direction r; false if it leaves the region.
+__device__ __forceinline__ bool region_step(int r, int &lx, int &ly) {
+  constexpr unsigned kDx = (0u << 0) | (0u << 2) | (1u << 4) | (2u << 6) |
+                           (2u << 8) | (2u << 10) | (1u << 12) | (0u << 14);
+  constexpr unsigned kDy = (1u << 0) | (0u << 2) | (0u << 4) | (0u << 6) |
+                           (1u << 8) | (2u << 10) | (2u << 12) | (2u << 14);
+  lx += static_cast<int>((kDx >> (2 * r)) & 3u) - 1;
+  ly += static_cast<int>((kDy >> (2 * r)) & 3u) - 1;
+  return lx >= 0 && lx < kReg && ly >= 0 && ly < kReg;
+}
+
+// Pass 1: level of every cell (receiver hops to an outlet), upstream cell
+// counts (ComputeFlowAcc, RB:262-279) and the level histogram (the level
+// sizes GenerateOrder produces, RB:213-254) for the tile's cells.  Levels
+// deeper than kHalo, or chains that leave the region, are reported through
+// hdr[3] and hdr[0]; the host then runs the level path for the step.
+__global__ __launch_bounds__(kAccBlock) void k_tile_acc(const uint8_t *__restrict__ rcv,
+                                                        uint32_t *__restrict__ cnt,
+                                                        int32_t *hdr, int W, int H, int tiles_x) {
+  __shared__ uint8_t r_s[kRegCells];
+  __shared__ uint8_t d_s[kRegCells];
+  __shared__ uint32_t c_s[kRegCells];
+  __shared__ uint16_t list[kRegCells];
+  __shared__ int lcount[256], loff[257], tcount[256];
+  __shared__ int s_maxl, s_tmax, s_escape;
+  const int x0 = (blockIdx.x % tiles_x) * kTile, y0 = (blockIdx.x / tiles_x) * kTile;
+  const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+    lcount[i] = 0;
+    tcount[i] = 0;
+  }
+  if (threadIdx.x == 0) {
+    s_maxl = 0;
+    s_tmax = 0;
+    s_escape = 0;
+  }
+  for (int i = threadIdx.x; i < kRegCells; i += blockDim.x) {
+    const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
+    r_s[i] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? rcv[gy * W + gx] : kOutside;
+  }
+  __syncthreads();
+  // level of each region cell by walking its chain inside the region
+  int maxl = 0;
+  for (int i = threadIdx.x; i < kRegCells; i += blockDim.x) {
+    uint8_t d = kEscaped;
+    if (r_s[i] != kOutside) {
+      int lx = i % kReg, ly = i / kReg, hops = 0;
+      while (true) {
+        const int r = r_s[ly * kReg + lx];
+        if (r >= 8) {                       // an outlet (or beyond the grid)
+          if (r != kOutside) d = static_cast<uint8_t>(hops);
+          break;
+        }
+        if (!region_step(r, lx, ly) || ++hops > 254) break;
+      }
+    }
+    d_s[i] = d;
+    if (d != kEscaped) {
+      atomicAdd(&lcount[d], 1);
+      maxl = max(maxl, static_cast<int>(d));
+    }
+  }
+  atomicMax(&s_maxl, maxl);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int L = 0; L <= s_maxl; L++) {
+      loff[L] = run;
+      run += lcount[L];
+      lcount[L] = loff[L];                // becomes the scatter cursor
+    }
+    loff[s_maxl + 1] = run;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kRegCells; i += blockDim.x) {
+    const uint8_t d = d_s[i];
+    if (d != kEscaped) list[atomicAdd(&lcount[d], 1)] = static_cast<uint16_t>(i);
+  }
+  __syncthreads();
+  // upstream counts, deepest level first: 1 + the counts of the donors,
+  // i.e. the neighbours whose receiver is this cell (pull form of RB:272-278)
+  for (int L = s_maxl; L >= 0; L--) {
+    for (int e = loff[L] + threadIdx.x; e < loff[L + 1]; e += blockDim.x) {
+      const int i = list[e];
+      const int lx = i % kReg, ly = i / kReg;
+      uint32_t s = 1;
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int nx = lx + ((j == 0 || j == 3 || j == 5) ? -1 : (j == 1 || j == 6) ? 0 : 1);
+        const int ny = ly + (j < 3 ? -1 : j < 5 ? 0 : 1);
+        if (nx < 0 || nx >= kReg || ny < 0 || ny >= kReg) continue;
+        // the neighbour in slot j drains here iff its direction is kSlotDir[j]
+        constexpr unsigned kSlotDir = (5u << 0) | (6u << 3) | (7u << 6) | (4u << 9) |
+                                      (0u << 12) | (3u << 15) | (2u << 18) | (1u << 21);
+        const int ni = ny * kReg + nx;
+        if (r_s[ni] == ((kSlotDir >> (3 * j)) & 7u)) s += c_s[ni];
+      }
+      c_s[i] = s;
+    }
+    __syncthreads();
+  }
+  // tile cells: counts, levels and the level histogram
+  int tmax = 0, esc = 0;
+  for (int i = threadIdx.x; i < kTile * kTile; i += blockDim.x) {
+    const int tx = i % kTile, ty = i / kTile;
+    const int gx = x0 + tx, gy = y0 + ty;
+    if (gx >= W || gy >= H) continue;
+    const int li = (ty + kHalo) * kReg + tx + kHalo;
+    const size_t g = static_cast<size_t>(gy) * W + gx;
+    const uint8_t d = d_s[li];
+    if (d == kEscaped) {
+      esc = 1;
+      continue;
+    }
+    cnt[g] = c_s[li];
+    atomicAdd(&tcount[d], 1);
+    tmax = max(tmax, static_cast<int>(d));
+  }
+  atomicMax(&s_tmax, tmax);
+  if (esc) atomicOr(&s_escape, 1);
+  __syncthreads();
+  int32_t *counts = hdr + kHdrInts;
+  for (int L = threadIdx.x; L <= s_tmax; L += blockDim.x)
+    if (tcount[L]) atomicAdd(&counts[L], tcount[L]);
+  if (threadIdx.x == 0) {
+    atomicMax(&hdr[0], s_tmax);
+    if (s_escape) atomicAdd(&hdr[3], 1);
+  }
+}
+
+// Pass 2: uplift and erosion (AddUplift RB:285-296, Erode RB:306-328) of the
+// tile's cells.  The cells on the tile cells' receiver chains are marked
+// with their level (all inside the region because every level is at most
+// kHalo), then processed level by level from the outlets up, each with the
+// reference's Newton iteration on LDS-resident elevations.  Reads h_old,
+// writes h_new: neighbouring tiles read this tile's cells as their halo.
+template <int NMODE>
+__global__ __launch_bounds__(kEroBlock) void k_tile_erode(
+    const double *__restrict__ h_old, double *__restrict__ h_new,
+    const uint8_t *__restrict__ rcv, const uint32_t *__restrict__ cnt,
+    const double *__restrict__ powtab, int W, int H, int tiles_x, ErodeConst k, int32_t *hdr) {
+  __shared__ double hs[kRegCells];
+  __shared__ uint8_t r_s[kRegCells];
+  __shared__ uint8_t d_s[kRegCells];
+  __shared__ uint16_t list[kRegCells];
+  __shared__ int lcount[kHalo + 2], loff[kHalo + 2];
+  __shared__ int s_maxl;
+  const int x0 = (blockIdx.x % tiles_x) * kTile, y0 = (blockIdx.x / tiles_x) * kTile;
+  const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
+  if (threadIdx.x < kHalo + 2) lcount[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_maxl = 0;
+  for (int i = threadIdx.x; i < kRegCells; i += blockDim.x) {
+    const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
+    const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
+    const size_t g = static_cast<size_t>(gy) * W + gx;
+    r_s[i] = in ? rcv[g] : kOutside;
+    hs[i] = in ? h_old[g] : 0.0;
+    d_s[i] = kEscaped;
+  }
+  __syncthreads();
+  // mark the chains of the tile cells with their levels
+  int maxl = 0;
+  for (int i = threadIdx.x; i < kTile * kTile; i += blockDim.x) {
+    const int tx = i % kTile, ty = i / kTile;
+    if (x0 + tx >= W || y0 + ty >= H) continue;
+    int lx = tx + kHalo, ly = ty + kHalo, len = 0;
+    bool ok = true;
+    while (true) {
+      const int r = r_s[ly * kReg + lx];
+      if (r >= 8) break;
+      if (!region_step(r, lx, ly) || ++len > kHalo) {
+        ok = false;
+        break;
+      }
+    }
+    if (!ok) {                 // cannot happen when pass 1 found every level <= kHalo
+      atomicOr(&hdr[2], kErrDepth);
+      continue;
+    }
+    maxl = max(maxl, len);
+    lx = tx + kHalo;
+    ly = ty + kHalo;
+    for (int d = len; d >= 0; d--) {
+      d_s[ly * kReg + lx] = static_cast<uint8_t>(d);
+      if (d > 0) region_step(r_s[ly * kReg + lx], lx, ly);
+    }
+  }
+  atomicMax(&s_maxl, maxl);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kRegCells; i += blockDim.x)
+    if (d_s[i] != kEscaped) atomicAdd(&lcount[d_s[i]], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int L = 0; L <= s_maxl; L++) {
+      loff[L] = run;
+      run += lcount[L];
+      lcount[L] = loff[L];
+    }
+    loff[s_maxl + 1] = run;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kRegCells; i += blockDim.x)
+    if (d_s[i] != kEscaped) list[atomicAdd(&lcount[d_s[i]], 1)] = static_cast<uint16_t>(i);
+  __syncthreads();
+  // level 0: uplift of interior outlets; rings 0 and 1 stay fixed
+  for (int e = loff[0] + threadIdx.x; e < loff[1]; e += blockDim.x) {
+    const int i = list[e];
+    const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
+    if (gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2) hs[i] += k.ud;
+  }
+  __syncthreads();
+  for (int L = 1; L <= s_maxl; L++) {
+    for (int e = loff[L] + threadIdx.x; e < loff[L + 1]; e += blockDim.x) {
+      const int i = list[e];
+      const int r = r_s[i];
+      int lx = i % kReg, ly = i / kReg;
+      const size_t g = static_cast<size_t>(ry0 + ly) * W + (rx0 + lx);
+      region_step(r, lx, ly);
+      const double fact = (k.kdt * powtab[cnt[g]]) / ((r & 1) ? k.lenpow1 : k.lenpow0);
+      hs[i] = newton_update<NMODE>(hs[i] + k.ud, hs[ly * kReg + lx], fact, k, hdr);
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < kTile * kTile; i += blockDim.x) {
+    const int tx = i % kTile, ty = i / kTile;
+    const int gx = x0 + tx, gy = y0 + ty;
+    if (gx >= W || gy >= H) continue;
+    h_new[static_cast<size_t>(gy) * W + gx] = hs[(ty + kHalo) * kReg + tx + kHalo];
   }
 }
 
@@ -660,8 +917,8 @@ struct fs_model {
   int nmode = 1;
   hipStream_t stream = nullptr;
 
-  // device state
-  double *h = nullptr;
+  // device state; h and h_alt swap after each tile-path step
+  double *h = nullptr, *h_alt = nullptr;
   uint8_t *rcv = nullptr, *dmask = nullptr;
   uint16_t *depth = nullptr;
   unsigned long long *jump = nullptr;
@@ -677,6 +934,9 @@ struct fs_model {
   std::vector<int32_t> counts, levels;
   int32_t nlevel = 0, maxdepth = 0, prev_maxdepth = FS_MAX_LEVELS;
   bool have_step = false;
+  bool order_current = false;      // stack and donor masks match rcv
+  int path = FS_PATH_AUTO;
+  uint64_t tile_steps = 0, level_steps = 0;
   unsigned long long ticket_base = 0;
   uint32_t epoch = 0;
 
@@ -768,7 +1028,8 @@ void free_model(fs_model *m) {
   if (!m) return;
   (void)hipSetDevice(m->device);
   if (m->stream) (void)hipStreamSynchronize(m->stream);
-  for (void *ptr : {static_cast<void *>(m->h), static_cast<void *>(m->rcv),
+  for (void *ptr : {static_cast<void *>(m->h), static_cast<void *>(m->h_alt),
+                    static_cast<void *>(m->rcv),
                     static_cast<void *>(m->dmask), static_cast<void *>(m->depth),
                     static_cast<void *>(m->jump), static_cast<void *>(m->list),
                     static_cast<void *>(m->stack), static_cast<void *>(m->hdr),
@@ -852,6 +1113,7 @@ int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_
   const size_t N = static_cast<size_t>(m->N);
   const size_t ntiles = (N + kBfsTile - 1) / kBfsTile + 1;
   FS_TRY(dev_alloc(&m->h, N));
+  FS_TRY(dev_alloc(&m->h_alt, N));
   FS_TRY(dev_alloc(&m->rcv, N + 16));
   FS_TRY(dev_alloc(&m->dmask, N));
   FS_TRY(dev_alloc(&m->depth, N));
@@ -967,35 +1229,63 @@ int launch_erosion(fs_model *m, const std::vector<Span> &spans) {
   return FS_OK;
 }
 
-// One time-step.
-int step_once(fs_model *m) {
+int launch_receivers(fs_model *m) {
+  const unsigned gx = static_cast<unsigned>((m->W - 4 + 63) / 64);
+  const unsigned gy = static_cast<unsigned>(std::min((m->H - 4 + 3) / 4, 65535));
+  FS_TRY(kt_begin(m, FS_K_RECEIVERS));
+  hipLaunchKernelGGL(k_receivers, dim3(gx, gy), dim3(256), 0, m->stream, m->h, m->rcv, m->W, m->H,
+                     m->d_card, m->d_diag);
+  FS_TRY(check_launch());
+  return kt_end(m);
+}
+
+int launch_fill_counts(fs_model *m) {
+  const int nclear = m->prev_maxdepth + 1;
+  hipLaunchKernelGGL(k_fill_counts, dim3((nclear + 255) / 256), dim3(256), 0, m->stream, m->hdr,
+                     nclear);
+  return check_launch();
+}
+
+// levels[] exactly as GenerateOrder leaves it (RB:218-251): offsets of
+// levels 0..D, the stack size, and the trailing duplicate; nlevel = D+3.
+// Reads the histogram from the header already fetched into hdr_h.
+int take_levels(fs_model *m, int D) {
+  if (D < 0 || D > FS_MAX_LEVELS) return FS_ERR_LEVEL_OVERFLOW;
+  if (D + 1 > kHdrCounts) FS_TRY(fetch_header(m, D + 1));
+  m->prev_maxdepth = D;
+  m->counts.assign(m->hdr_h + kHdrInts, m->hdr_h + kHdrInts + D + 1);
+  long long total = 0;
+  for (int32_t c : m->counts) total += c;
+  if (total != m->N) {
+    fs::set_last_error("level histogram covers " + std::to_string(total) + " of " +
+                       std::to_string(m->N) + " cells");
+    return m->hdr_h[1] > 0 ? FS_ERR_LEVEL_OVERFLOW : FS_ERR_INTERNAL;
+  }
+  m->levels.resize(D + 3);
+  m->levels[0] = 0;
+  for (int L = 0; L <= D; L++) m->levels[L + 1] = m->levels[L] + m->counts[L];
+  m->levels[D + 2] = m->N;
+  m->nlevel = D + 3;
+  std::memcpy(m->levels_h, m->levels.data(), m->levels.size() * sizeof(int32_t));
+  FS_HIP_TRY(hipMemcpyAsync(m->levels_d, m->levels_h, m->levels.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice, m->stream));
+  return FS_OK;
+}
+
+// ComputeDonors + GenerateOrder (RB:188-254) from the receivers in rcv:
+// donor masks, levels (with pointer jumping for deep cells), the level
+// histogram, and the exact stack.  Synchronises once (twice with deep cells).
+int build_order(fs_model *m, hipEvent_t ev_topology_done) {
   hipStream_t s = m->stream;
-  hipEvent_t *ev = m->ev[m->ev_cur];
   const int W = m->W, H = m->H;
   const int32_t N = m->N;
-
-  FS_HIP_TRY(hipEventRecord(ev[0], s));
-  {
-    const unsigned gx = static_cast<unsigned>((W - 4 + 63) / 64);
-    const unsigned gy = static_cast<unsigned>(std::min((H - 4 + 3) / 4, 65535));
-    FS_TRY(kt_begin(m, FS_K_RECEIVERS));
-    hipLaunchKernelGGL(k_receivers, dim3(gx, gy), dim3(256), 0, s, m->h, m->rcv, W, H,
-                       m->d_card, m->d_diag);
-    FS_TRY(check_launch());
-    FS_TRY(kt_end(m));
-  }
-  FS_HIP_TRY(hipEventRecord(ev[1], s));
-  {
-    const int nclear = m->prev_maxdepth + 1;
-    hipLaunchKernelGGL(k_fill_counts, dim3((nclear + 255) / 256), dim3(256), 0, s, m->hdr, nclear);
-    FS_TRY(check_launch());
-    FS_TRY(kt_begin(m, FS_K_TOPOLOGY));
-    hipLaunchKernelGGL(k_topology, dim3(std::min(H, kTopoBlocks)), dim3(256), 0, s, m->rcv,
-                       m->dmask, m->depth, m->jump, m->list, m->hdr, W, H);
-    FS_TRY(check_launch());
-    FS_TRY(kt_end(m));
-  }
-  FS_HIP_TRY(hipEventRecord(ev[2], s));
+  FS_TRY(launch_fill_counts(m));
+  FS_TRY(kt_begin(m, FS_K_TOPOLOGY));
+  hipLaunchKernelGGL(k_topology, dim3(std::min(H, kTopoBlocks)), dim3(256), 0, s, m->rcv,
+                     m->dmask, m->depth, m->jump, m->list, m->hdr, W, H);
+  FS_TRY(check_launch());
+  FS_TRY(kt_end(m));
+  if (ev_topology_done) FS_HIP_TRY(hipEventRecord(ev_topology_done, s));
   FS_TRY(fetch_header(m, kHdrCounts));
   FS_TRY(stage_collect(m, m->ev_cur ^ 1));
   FS_TRY(kt_collect(m));
@@ -1012,29 +1302,7 @@ int step_once(fs_model *m) {
     FS_TRY(device_error_status(m->hdr_h[2]));
   }
   const int D = m->hdr_h[0];
-  if (D < 0 || D > FS_MAX_LEVELS) return FS_ERR_LEVEL_OVERFLOW;
-  if (D + 1 > kHdrCounts) FS_TRY(fetch_header(m, D + 1));
-  m->prev_maxdepth = D;
-  m->counts.assign(m->hdr_h + kHdrInts, m->hdr_h + kHdrInts + D + 1);
-  long long total = 0;
-  for (int32_t c : m->counts) total += c;
-  if (total != N) {
-    fs::set_last_error("level histogram covers " + std::to_string(total) + " of " +
-                       std::to_string(N) + " cells");
-    return m->hdr_h[1] > 0 ? FS_ERR_LEVEL_OVERFLOW : FS_ERR_INTERNAL;
-  }
-  // levels[] exactly as GenerateOrder leaves it (RB:218-251): offsets of
-  // levels 0..D, the stack size, and the trailing duplicate; nlevel = D+3.
-  m->levels.resize(D + 3);
-  m->levels[0] = 0;
-  for (int L = 0; L <= D; L++) m->levels[L + 1] = m->levels[L] + m->counts[L];
-  m->levels[D + 2] = N;
-  m->nlevel = D + 3;
-  std::memcpy(m->levels_h, m->levels.data(), m->levels.size() * sizeof(int32_t));
-  FS_HIP_TRY(hipMemcpyAsync(m->levels_d, m->levels_h, m->levels.size() * sizeof(int32_t),
-                            hipMemcpyHostToDevice, s));
-
-  // ---- GenerateOrder ----
+  FS_TRY(take_levels(m, D));
   {
     const unsigned tiles = static_cast<unsigned>((N + kRootTile - 1) / kRootTile);
     FS_TRY(kt_begin(m, FS_K_ROOTS));
@@ -1064,8 +1332,20 @@ int step_once(fs_model *m) {
       }
     }
   }
-  FS_HIP_TRY(hipEventRecord(ev[3], s));
+  m->order_current = true;
+  return FS_OK;
+}
 
+// The level path: the reference's stage sequence with each level a launch
+// (or a run of small levels one single-workgroup launch).  Receivers are
+// already in rcv.
+int level_path(fs_model *m, hipEvent_t *ev) {
+  hipStream_t s = m->stream;
+  const int W = m->W, H = m->H;
+  const int32_t N = m->N;
+  FS_TRY(build_order(m, ev[2]));
+  const int D = static_cast<int>(m->counts.size()) - 1;
+  FS_HIP_TRY(hipEventRecord(ev[3], s));
   // ---- FlowAcc: levels D down to 0 ----
   {
     std::vector<Span> spans = plan_levels(m->counts, 0, D);
@@ -1087,7 +1367,6 @@ int step_once(fs_model *m) {
     }
   }
   FS_HIP_TRY(hipEventRecord(ev[4], s));
-
   // ---- Uplift of the interior outlets ----
   {
     const int32_t n0 = m->counts[0];
@@ -1098,7 +1377,6 @@ int step_once(fs_model *m) {
     FS_TRY(kt_end(m));
   }
   FS_HIP_TRY(hipEventRecord(ev[5], s));
-
   // ---- Erode: levels 1..D ----
   if (D >= 1) {
     const std::vector<Span> spans = plan_levels(m->counts, 1, D);
@@ -1106,7 +1384,67 @@ int step_once(fs_model *m) {
     else if (m->nmode == 2) FS_TRY(launch_erosion<2>(m, spans));
     else FS_TRY(launch_erosion<0>(m, spans));
   }
-  FS_HIP_TRY(hipEventRecord(ev[6], s));
+  m->level_steps++;
+  return FS_OK;
+}
+
+template <int NMODE>
+int launch_tile_erode(fs_model *m, int tiles, int tiles_x) {
+  hipLaunchKernelGGL(k_tile_erode<NMODE>, dim3(tiles), dim3(kEroBlock), 0, m->stream, m->h, m->h_alt,
+                     m->rcv, m->cnt, m->powtab, m->W, m->H, tiles_x, m->ek, m->hdr);
+  return check_launch();
+}
+
+// The tile path (every level <= kHalo): two region kernels and one header
+// read.  Returns FS_OK with *done = false when the levels are too deep; the
+// caller then runs the level path on the same receivers.
+int tile_path(fs_model *m, hipEvent_t *ev, bool *done) {
+  *done = false;
+  hipStream_t s = m->stream;
+  const int tiles_x = (m->W + kTile - 1) / kTile, tiles_y = (m->H + kTile - 1) / kTile;
+  const int tiles = tiles_x * tiles_y;
+  FS_TRY(launch_fill_counts(m));
+  FS_TRY(kt_begin(m, FS_K_TILE_ACC));
+  hipLaunchKernelGGL(k_tile_acc, dim3(tiles), dim3(kAccBlock), 0, s, m->rcv, m->cnt, m->hdr, m->W,
+                     m->H, tiles_x);
+  FS_TRY(check_launch());
+  FS_TRY(kt_end(m));
+  FS_HIP_TRY(hipEventRecord(ev[2], s));
+  FS_TRY(fetch_header(m, kHdrCounts));
+  FS_TRY(stage_collect(m, m->ev_cur ^ 1));
+  FS_TRY(kt_collect(m));
+  FS_TRY(device_error_status(m->hdr_h[2]));
+  const int D = m->hdr_h[0];
+  if (m->hdr_h[3] != 0 || D > kHalo) {
+    m->prev_maxdepth = std::max(m->prev_maxdepth, std::min(D, FS_MAX_LEVELS));
+    return FS_OK;
+  }
+  FS_TRY(take_levels(m, D));
+  m->order_current = false;                 // stack / donor masks rebuilt on request
+  FS_HIP_TRY(hipEventRecord(ev[3], s));
+  FS_HIP_TRY(hipEventRecord(ev[4], s));
+  FS_HIP_TRY(hipEventRecord(ev[5], s));
+  FS_TRY(kt_begin(m, FS_K_TILE_ERODE));
+  if (m->nmode == 1) FS_TRY(launch_tile_erode<1>(m, tiles, tiles_x));
+  else if (m->nmode == 2) FS_TRY(launch_tile_erode<2>(m, tiles, tiles_x));
+  else FS_TRY(launch_tile_erode<0>(m, tiles, tiles_x));
+  FS_TRY(kt_end(m));
+  std::swap(m->h, m->h_alt);
+  m->tile_steps++;
+  *done = true;
+  return FS_OK;
+}
+
+// One time-step.
+int step_once(fs_model *m) {
+  hipEvent_t *ev = m->ev[m->ev_cur];
+  FS_HIP_TRY(hipEventRecord(ev[0], m->stream));
+  FS_TRY(launch_receivers(m));
+  FS_HIP_TRY(hipEventRecord(ev[1], m->stream));
+  bool done = false;
+  if (m->path == FS_PATH_AUTO) FS_TRY(tile_path(m, ev, &done));
+  if (!done) FS_TRY(level_path(m, ev));
+  FS_HIP_TRY(hipEventRecord(ev[6], m->stream));
   m->ev_pending[m->ev_cur] = true;
   m->ev_cur ^= 1;
   m->have_step = true;
@@ -1159,6 +1497,16 @@ int check_model(fs_model *m) {
     return FS_ERR_HIP;
   }
   return FS_OK;
+}
+
+// Rebuild the stack and donor masks of the last step if it took the tile path.
+int ensure_order(fs_model *m) {
+  if (m->order_current || !m->have_step) return FS_OK;
+  FS_HIP_TRY(hipStreamSynchronize(m->stream));
+  FS_TRY(build_order(m, nullptr));
+  FS_HIP_TRY(hipStreamSynchronize(m->stream));
+  FS_TRY(fetch_header(m, 0));
+  return device_error_status(m->hdr_h[2]);
 }
 
 template <class T>
@@ -1260,6 +1608,7 @@ int fs_get_donors(fs_model *m, int32_t *ndon, int32_t *donor, size_t count) {
   FS_TRY(check_model(m));
   if (!ndon || !donor) return FS_ERR_INVALID_ARGUMENT;
   if (count != static_cast<size_t>(m->N)) return FS_ERR_BUFFER_SIZE;
+  FS_TRY(ensure_order(m));
   std::vector<uint8_t> mk(count);
   FS_TRY(download(m, mk.data(), m->dmask, count));
   const int W = m->W;
@@ -1278,6 +1627,7 @@ int fs_get_stack(fs_model *m, int32_t *stack, size_t count) {
   FS_TRY(check_model(m));
   if (!stack) return FS_ERR_INVALID_ARGUMENT;
   if (count != static_cast<size_t>(m->N)) return FS_ERR_BUFFER_SIZE;
+  FS_TRY(ensure_order(m));
   return download(m, stack, m->stack, count);
 }
 
@@ -1370,10 +1720,24 @@ int fs_calibrate_copy(size_t nbytes, int device, double *ms) {
   return st;
 }
 
+int fs_set_step_path(fs_model *m, int path) {
+  if (!m || (path != FS_PATH_AUTO && path != FS_PATH_LEVELS)) return FS_ERR_INVALID_ARGUMENT;
+  m->path = path;
+  return FS_OK;
+}
+
+int fs_path_stats(fs_model *m, uint64_t *tile_steps, uint64_t *level_steps) {
+  if (!m) return FS_ERR_INVALID_ARGUMENT;
+  if (tile_steps) *tile_steps = m->tile_steps;
+  if (level_steps) *level_steps = m->level_steps;
+  return FS_OK;
+}
+
 const char *fs_kernel_name(int kernel) {
-  static const char *names[FS_NUM_KERNELS] = {"k_receivers", "k_topology", "k_roots", "k_bfs",
-                                              "k_flowacc",   "k_uplift_roots", "k_erode",
-                                              "k_jump"};
+  static const char *names[FS_NUM_KERNELS] = {"k_receivers", "k_topology",   "k_roots",
+                                              "k_bfs",       "k_flowacc",    "k_uplift_roots",
+                                              "k_erode",     "k_jump",       "k_tile_acc",
+                                              "k_tile_erode"};
   if (kernel < 0 || kernel >= FS_NUM_KERNELS) return "unknown";
   return names[kernel];
 }

@@ -4,10 +4,11 @@ The compute path is the HIP library (csrc/); this package only loads it and
 mirrors the reference class surface (fastscape_RB.cpp:46-406) for tests,
 benchmarks and scripting.
 """
-from ._lib import (EXPORTED, FS_MAX_LEVELS, FS_NO_FLOW, KERNELS, LIB_PATH, PRESET_BENCHMARK,
-                   PRESET_REFERENCE, TIMER_NAMES, FastScape, FastScapeError, FsParams,
+from ._lib import (EXPORTED, FS_MAX_LEVELS, FS_NO_FLOW, KERNELS, LIB_PATH, PATH_AUTO,
+                   PATH_LEVELS, PRESET_BENCHMARK, PRESET_REFERENCE, TILE_HALO, TIMER_NAMES, FastScape, FastScapeError, FsParams,
                    calibrate_copy, generate_terrain, library, preset, write_dem)
 
-__all__ = ["EXPORTED", "FS_MAX_LEVELS", "FS_NO_FLOW", "KERNELS", "LIB_PATH",
+__all__ = ["EXPORTED", "FS_MAX_LEVELS", "FS_NO_FLOW", "KERNELS", "LIB_PATH", "PATH_AUTO",
+           "PATH_LEVELS", "TILE_HALO",
            "PRESET_BENCHMARK", "PRESET_REFERENCE", "TIMER_NAMES", "FastScape",
            "FastScapeError", "FsParams", "calibrate_copy", "generate_terrain", "library", "preset", "write_dem"]

@@ -22,14 +22,17 @@ TIMER_NAMES = ["Step1: Initialize", "Step2: DetermineReceivers", "Step3: Determi
                "Step4: GenerateOrder", "Step5: FlowAcc", "Step6: Uplift", "Step7: Erosion",
                "Overall"]
 KERNELS = ["k_receivers", "k_topology", "k_roots", "k_bfs", "k_flowacc", "k_uplift_roots",
-           "k_erode", "k_jump"]
+           "k_erode", "k_jump", "k_tile_acc", "k_tile_erode"]
+PATH_AUTO = 0
+PATH_LEVELS = 1
+TILE_HALO = 16
 
 # Every symbol declared in include/fastscape.h.
 EXPORTED = [
     "fs_params_preset", "fs_create", "fs_create_from_h", "fs_destroy", "fs_run", "fs_step",
     "fs_get_h", "fs_set_h", "fs_get_timers", "fs_reset_timers", "fs_get_receivers",
     "fs_get_donors", "fs_get_stack", "fs_get_levels", "fs_get_accum", "fs_device_h",
-    "fs_stream", "fs_synchronize", "fs_set_kernel_timing", "fs_kernel_stats",
+    "fs_stream", "fs_synchronize", "fs_set_step_path", "fs_path_stats", "fs_set_kernel_timing", "fs_kernel_stats",
     "fs_kernel_name", "fs_calibrate_copy", "fs_write_dem", "fs_generate_terrain", "fs_strerror", "fs_last_error",
 ]
 
@@ -88,6 +91,8 @@ def _lib():
         "fs_device_h": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
         "fs_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
         "fs_synchronize": (ctypes.c_int, [_vp]),
+        "fs_set_step_path": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "fs_path_stats": (ctypes.c_int, [_vp, _u64p, _u64p]),
         "fs_set_kernel_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
         "fs_kernel_stats": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _u64p]),
         "fs_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
@@ -273,6 +278,16 @@ class FastScape:
         p = _vp()
         _check(_lib().fs_device_h(self._m, ctypes.byref(p)), "fs_device_h")
         return p.value
+
+    def set_path(self, path):
+        """PATH_AUTO (tile path when levels allow) or PATH_LEVELS."""
+        _check(_lib().fs_set_step_path(self._m, int(path)), "fs_set_step_path")
+
+    def path_stats(self):
+        t = ctypes.c_uint64(0)
+        lv = ctypes.c_uint64(0)
+        _check(_lib().fs_path_stats(self._m, ctypes.byref(t), ctypes.byref(lv)), "fs_path_stats")
+        return {"tile": t.value, "levels": lv.value}
 
     def set_kernel_timing(self, enable):
         _check(_lib().fs_set_kernel_timing(self._m, 1 if enable else 0), "fs_set_kernel_timing")

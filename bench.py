@@ -37,6 +37,10 @@ KERNEL_BYTES = {
     "k_bfs": 9,            # stack entry 4 + donor mask 1 + donor index out 4
     "k_flowacc": 13,       # stack 4 + mask 1 + donor counts 4 (one donor/cell) + count 4
     "k_erode": 33,         # stack 4 + rcv 1 + count 4 + h 8 + receiver h 8 + h out 8
+    # tile path, per cell, at the survey's per-stage figures (79 B/cell-step in
+    # total): donors 2 + order 10 + accumulation 21; uplift + erosion 37
+    "k_tile_acc": 33,
+    "k_tile_erode": 37,
 }
 
 
@@ -111,9 +115,10 @@ def main():
     stats = m.kernel_stats()
     m.set_kernel_timing(False)
     levels = m.levels()
+    paths = m.path_stats()
     n0 = int(levels[1]) if levels.size > 1 else N
     units = {"k_receivers": (W - 4) * (H - 4), "k_topology": N, "k_bfs": N - n0,
-             "k_flowacc": N, "k_erode": N - n0}
+             "k_flowacc": N, "k_erode": N - n0, "k_tile_acc": N, "k_tile_erode": N}
     dom = max(stats, key=lambda k: stats[k][0])
     ms, launches = stats[dom]
     per_step_bytes = KERNEL_BYTES.get(dom, 0) * units.get(dom, N)
@@ -134,7 +139,8 @@ def main():
             "data": "synthetic (mt19937 seed 0 uniform[0,1) terrain, rings 0/1 at 0)",
             "config": {"workload": "FastScape %dx%d FP64 time-steps, K=1e-5 n=1 m=0.4 U=2e-3 "
                                    "dt=1000 dx=100" % (W, H),
-                       "grid": [W, H], "nlevel_last_step": int(levels.size)},
+                       "grid": [W, H], "nlevel_last_step": int(levels.size),
+                       "step_paths": paths},
             "roofline": roofline, "cpu_baseline": cpu,
             "stage_kernel_ms_per_step": {k: round(v[0] / args.profile_steps, 4)
                                          for k, v in stats.items()}}

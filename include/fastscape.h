@@ -125,11 +125,30 @@ int fs_set_h(fs_model *m, const double *in, size_t count);
 
 /* Per-stage times in microseconds, in FS_T_* order (RB:380-387).  Stage
  * times are device time between HIP events bracketing each stage; Overall is
- * host wall time of fs_run / fs_step plus construction. */
+ * host wall time of fs_run / fs_step plus construction.  On the tile path
+ * (below) donors, levels and accumulation are one kernel, reported under
+ * DetermineDonors; GenerateOrder is the level-histogram read-back; uplift is
+ * fused into Erosion.  FS_PATH_LEVELS gives the reference's stage split. */
 int fs_get_timers(fs_model *m, uint64_t us[FS_NUM_TIMERS]);
 int fs_reset_timers(fs_model *m);
 
+/* ---- step path ------------------------------------------------------------
+ * FS_PATH_AUTO (default): when every cell is at most FS_TILE_HALO receiver
+ * hops from its outlet, accumulation and erosion run per 64x64 tile with a
+ * 16-cell halo in LDS (two kernels per step); otherwise the step falls back
+ * to the level path.  FS_PATH_LEVELS: always the level path -- one launch
+ * per level for order, accumulation and erosion, as the reference's stages
+ * (RB:366-372).  Both give bit-identical results.  No reference counterpart. */
+#define FS_PATH_AUTO   0
+#define FS_PATH_LEVELS 1
+#define FS_TILE_HALO   16
+int fs_set_step_path(fs_model *m, int path);
+/* Number of time-steps taken by each path since creation. */
+int fs_path_stats(fs_model *m, uint64_t *tile_steps, uint64_t *level_steps);
+
 /* ---- state of the last completed time-step (parity checks) -------------- */
+/* After a tile-path step the stack and donor lists are rebuilt from that
+ * step's receivers on the first request (same kernels as the level path). */
 /* rec[] in the reference encoding: 0..7 or FS_NO_FLOW (RB:86, 160-182). */
 int fs_get_receivers(fs_model *m, int32_t *rec, size_t count);
 /* ndon[count] and donor[8*count] as ComputeDonors leaves them
@@ -164,7 +183,9 @@ int fs_synchronize(fs_model *m);
 #define FS_K_UPLIFT    5
 #define FS_K_EROSION   6
 #define FS_K_JUMP      7
-#define FS_NUM_KERNELS 8
+#define FS_K_TILE_ACC  8
+#define FS_K_TILE_ERODE 9
+#define FS_NUM_KERNELS 10
 int fs_set_kernel_timing(fs_model *m, int enable);
 int fs_kernel_stats(fs_model *m, int kernel, double *total_ms, uint64_t *launches);
 const char *fs_kernel_name(int kernel);

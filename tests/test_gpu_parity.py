@@ -274,3 +274,43 @@ def test_large_grid_invariants(oracle):
         m.step(1)
         compare_step(m, st, "4096 step")
         np.testing.assert_array_equal(m.getH().ravel(), st["h"])
+
+
+@pytest.mark.parametrize("W,H,preset,steps", [(1000, 800, 1, 30), (130, 70, 1, 40),
+                                              (300, 300, 0, 60)])
+def test_tile_path_equals_level_path(oracle, W, H, preset, steps):
+    """The tile path and the level path are bit-identical step after step;
+    the reference preset deepens past the tile halo and falls back."""
+    note("tile vs levels %dx%d preset %d" % (W, H, preset))
+    h0 = oracle.init_terrain(W, H, 0)
+    with fa.FastScape(W, H, gpu_params(preset), h=h0) as a, \
+            fa.FastScape(W, H, gpu_params(preset), h=h0) as b:
+        b.set_path(fa.PATH_LEVELS)
+        for s in range(steps):
+            a.step(1)
+            b.step(1)
+            if s % 10 == 9 or s == steps - 1:
+                np.testing.assert_array_equal(a.getH(), b.getH(), err_msg="h step %d" % s)
+                np.testing.assert_array_equal(a.accum(), b.accum(), err_msg="accum %d" % s)
+                np.testing.assert_array_equal(a.levels(), b.levels(), err_msg="levels %d" % s)
+                np.testing.assert_array_equal(a.stack(), b.stack(), err_msg="stack %d" % s)
+        pa, pb = a.path_stats(), b.path_stats()
+        assert pb == {"tile": 0, "levels": steps}
+        assert pa["tile"] + pa["levels"] == steps
+        if preset == 1:
+            assert pa["tile"] == steps            # shallow drainage: always tiles
+        else:
+            assert pa["levels"] > 0                # deep drainage: falls back
+
+
+def test_benchmark_grid_takes_tile_path(oracle):
+    note("tile path at 2000^2")
+    W = H = 2000
+    with fa.FastScape(W, H, gpu_params(1), engine_seed=0) as m:
+        m.step(5)
+        assert m.path_stats() == {"tile": 5, "levels": 0}
+        h = m.getH().ravel().copy()
+        st = oracle.step(h, W, H, oracle_params(oracle, 1))
+        m.step(1)
+        compare_step(m, st, "2000 tile step")
+        np.testing.assert_array_equal(m.getH().ravel(), st["h"])
