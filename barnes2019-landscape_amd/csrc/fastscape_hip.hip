@@ -661,220 +661,186 @@ __device__ __forceinline__ bool region_step(int r, int &lx, int &ly) {
   return lx >= 0 && lx < kReg && ly >= 0 && ly < kReg;
 }
 
-// Pass 1: level of every cell (receiver hops to an outlet), upstream cell
-// counts (ComputeFlowAcc, RB:262-279) and the level histogram (the level
-// sizes GenerateOrder produces, RB:213-254) for the tile's cells.  Levels
-// deeper than kHalo, or chains that leave the region, are reported through
-// hdr[3] and hdr[0]; the host then runs the level path for the step.
-__global__ __launch_bounds__(kAccBlock) void k_tile_acc(const uint8_t *__restrict__ rcv,
-                                                        uint32_t *__restrict__ cnt,
-                                                        int32_t *hdr, int W, int H, int tiles_x) {
-  __shared__ uint8_t r_s[kRegCells];
-  __shared__ uint8_t d_s[kRegCells];
-  __shared__ uint32_t c_s[kRegCells];
-  __shared__ uint16_t list[kRegCells];
-  __shared__ int lcount[256], loff[257], tcount[256];
-  __shared__ int s_maxl, s_tmax, s_escape;
-  const int x0 = (blockIdx.x % tiles_x) * kTile, y0 = (blockIdx.x / tiles_x) * kTile;
-  const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
-  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
-    lcount[i] = 0;
-    tcount[i] = 0;
-  }
-  if (threadIdx.x == 0) {
-    s_maxl = 0;
-    s_tmax = 0;
-    s_escape = 0;
-  }
-  for (int i = threadIdx.x; i < kRegCells; i += blockDim.x) {
-    const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
-    r_s[i] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? rcv[gy * W + gx] : kOutside;
-  }
-  __syncthreads();
-  // level of each region cell by walking its chain inside the region
-  int maxl = 0;
-  for (int i = threadIdx.x; i < kRegCells; i += blockDim.x) {
-    uint8_t d = kEscaped;
-    if (r_s[i] != kOutside) {
-      int lx = i % kReg, ly = i / kReg, hops = 0;
-      while (true) {
-        const int r = r_s[ly * kReg + lx];
-        if (r >= 8) {                       // an outlet (or beyond the grid)
-          if (r != kOutside) d = static_cast<uint8_t>(hops);
-          break;
-        }
-        if (!region_step(r, lx, ly) || ++hops > 254) break;
-      }
-    }
-    d_s[i] = d;
-    if (d != kEscaped) {
-      atomicAdd(&lcount[d], 1);
-      maxl = max(maxl, static_cast<int>(d));
-    }
-  }
-  atomicMax(&s_maxl, maxl);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int L = 0; L <= s_maxl; L++) {
-      loff[L] = run;
-      run += lcount[L];
-      lcount[L] = loff[L];                // becomes the scatter cursor
-    }
-    loff[s_maxl + 1] = run;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < kRegCells; i += blockDim.x) {
-    const uint8_t d = d_s[i];
-    if (d != kEscaped) list[atomicAdd(&lcount[d], 1)] = static_cast<uint16_t>(i);
-  }
-  __syncthreads();
-  // upstream counts, deepest level first: 1 + the counts of the donors,
-  // i.e. the neighbours whose receiver is this cell (pull form of RB:272-278)
-  for (int L = s_maxl; L >= 0; L--) {
-    for (int e = loff[L] + threadIdx.x; e < loff[L + 1]; e += blockDim.x) {
-      const int i = list[e];
-      const int lx = i % kReg, ly = i / kReg;
-      uint32_t s = 1;
+// Workgroups are dealt to the 8 XCDs round-robin by block index; hand each
+// XCD one contiguous run of tiles so that tiles sharing halos share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
+  const int q = ntiles / 8, rem = ntiles % 8;
+  const int x = b % 8, k = b / 8;
+  return x * q + min(x, rem) + k;
+}
+
+// Load the receivers (and optionally elevations and levels) of the region:
+// every load of a thread is issued before the first LDS store.
+template <int BLOCK, bool WITH_H>
+__device__ __forceinline__ void load_region(const uint8_t *__restrict__ rcv,
+                                            const double *__restrict__ h,
+                                            const uint8_t *__restrict__ lvl, uint8_t *r_s,
+                                            double *hs, uint8_t *d_s, int rx0, int ry0, int W,
+                                            int H) {
+  constexpr int kPer = kRegCells / BLOCK;
+  static_assert(kRegCells % BLOCK == 0, "region must split evenly over the block");
+  uint8_t rv[kPer], dv[WITH_H ? kPer : 1];
+  double hv[WITH_H ? kPer : 1];
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int nx = lx + ((j == 0 || j == 3 || j == 5) ? -1 : (j == 1 || j == 6) ? 0 : 1);
-        const int ny = ly + (j < 3 ? -1 : j < 5 ? 0 : 1);
-        if (nx < 0 || nx >= kReg || ny < 0 || ny >= kReg) continue;
-        // the neighbour in slot j drains here iff its direction is kSlotDir[j]
-        constexpr unsigned kSlotDir = (5u << 0) | (6u << 3) | (7u << 6) | (4u << 9) |
-                                      (0u << 12) | (3u << 15) | (2u << 18) | (1u << 21);
-        const int ni = ny * kReg + nx;
-        if (r_s[ni] == ((kSlotDir >> (3 * j)) & 7u)) s += c_s[ni];
-      }
-      c_s[i] = s;
+  for (int k = 0; k < kPer; k++) {
+    const int i = k * BLOCK + threadIdx.x;
+    const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
+    const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
+    const size_t g = in ? static_cast<size_t>(gy) * W + gx : 0;
+    rv[k] = in ? rcv[g] : kOutside;
+    if (WITH_H) {
+      hv[k] = in ? h[g] : 0.0;
+      dv[k] = in ? lvl[g] : kEscaped;
     }
-    __syncthreads();
   }
-  // tile cells: counts, levels and the level histogram
-  int tmax = 0, esc = 0;
-  for (int i = threadIdx.x; i < kTile * kTile; i += blockDim.x) {
-    const int tx = i % kTile, ty = i / kTile;
-    const int gx = x0 + tx, gy = y0 + ty;
-    if (gx >= W || gy >= H) continue;
-    const int li = (ty + kHalo) * kReg + tx + kHalo;
-    const size_t g = static_cast<size_t>(gy) * W + gx;
-    const uint8_t d = d_s[li];
-    if (d == kEscaped) {
-      esc = 1;
-      continue;
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const int i = k * BLOCK + threadIdx.x;
+    r_s[i] = rv[k];
+    if (WITH_H) {
+      hs[i] = hv[k];
+      d_s[i] = dv[k];
     }
-    cnt[g] = c_s[li];
-    atomicAdd(&tcount[d], 1);
-    tmax = max(tmax, static_cast<int>(d));
-  }
-  atomicMax(&s_tmax, tmax);
-  if (esc) atomicOr(&s_escape, 1);
-  __syncthreads();
-  int32_t *counts = hdr + kHdrInts;
-  for (int L = threadIdx.x; L <= s_tmax; L += blockDim.x)
-    if (tcount[L]) atomicAdd(&counts[L], tcount[L]);
-  if (threadIdx.x == 0) {
-    atomicMax(&hdr[0], s_tmax);
-    if (s_escape) atomicAdd(&hdr[3], 1);
   }
 }
 
+// Pass 1: upstream cell counts (ComputeFlowAcc, RB:262-279) and levels
+// (receiver hops to an outlet, the level GenerateOrder assigns, RB:213-254)
+// of the tile's cells.  Every region cell walks its receiver chain once and
+// adds 1 to each cell it passes, itself included, so a cell's count is the
+// number of region cells draining through it -- its whole upstream area when
+// every level is at most kHalo.  A tile cell deeper than that (or whose chain
+// leaves the region) raises hdr[3]; the host then runs the level path.
+__global__ __launch_bounds__(kAccBlock) void k_tile_acc(const uint8_t *__restrict__ rcv,
+                                                        uint32_t *__restrict__ cnt,
+                                                        uint8_t *__restrict__ lvl, int32_t *hdr,
+                                                        int W, int H, int tiles_x, int ntiles) {
+  __shared__ uint8_t r_s[kRegCells];
+  __shared__ uint32_t c_s[kRegCells];
+  __shared__ int s_over;
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
+  const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
+  if (threadIdx.x == 0) s_over = 0;
+  for (int it = 0; it < kRegCells / kAccBlock; it++) c_s[it * kAccBlock + threadIdx.x] = 0;
+  load_region<kAccBlock, false>(rcv, nullptr, nullptr, r_s, nullptr, nullptr, rx0, ry0, W, H);
+  __syncthreads();
+  int over = 0;
+  for (int it = 0; it < kRegCells / kAccBlock; it++) {
+    const int i = it * kAccBlock + threadIdx.x;
+    if (r_s[i] == kOutside) continue;
+    int lx = i % kReg, ly = i / kReg, hops = 0;
+    bool rooted = false;
+    while (true) {
+      const int x = ly * kReg + lx;
+      atomicAdd(&c_s[x], 1u);
+      const int r = r_s[x];
+      if (r >= 8) {
+        rooted = true;
+        break;
+      }
+      if (!region_step(r, lx, ly) || ++hops > kHalo) break;
+    }
+    // the level of a tile cell is the length of its walk
+    const int tx = i % kReg - kHalo, ty = i / kReg - kHalo;
+    if (tx >= 0 && tx < kTile && ty >= 0 && ty < kTile) {
+      const size_t g = static_cast<size_t>(y0 + ty) * W + (x0 + tx);
+      if (rooted) lvl[g] = static_cast<uint8_t>(hops);
+      else over = 1;
+    }
+  }
+  if (over) s_over = 1;
+  __syncthreads();
+  for (int it = 0; it < kTile * kTile / kAccBlock; it++) {
+    const int i = it * kAccBlock + threadIdx.x;
+    const int tx = i % kTile, ty = i / kTile;
+    const int gx = x0 + tx, gy = y0 + ty;
+    if (gx >= W || gy >= H) continue;
+    cnt[static_cast<size_t>(gy) * W + gx] = c_s[(ty + kHalo) * kReg + tx + kHalo];
+  }
+  if (threadIdx.x == 0 && s_over) atomicAdd(&hdr[3], 1);
+}
+
 // Pass 2: uplift and erosion (AddUplift RB:285-296, Erode RB:306-328) of the
-// tile's cells.  The cells on the tile cells' receiver chains are marked
-// with their level (all inside the region because every level is at most
-// kHalo), then processed level by level from the outlets up, each with the
-// reference's Newton iteration on LDS-resident elevations.  Reads h_old,
-// writes h_new: neighbouring tiles read this tile's cells as their halo.
+// tile's cells.  Levels come from pass 1; every region cell is processed
+// level by level from the outlets up with the reference's Newton iteration
+// on LDS-resident elevations (the tile cells' receiver chains lie in the
+// region because every level is at most kHalo).  Reads h_old, writes h_new:
+// neighbouring tiles read this tile's cells as their halo.
 template <int NMODE>
 __global__ __launch_bounds__(kEroBlock) void k_tile_erode(
     const double *__restrict__ h_old, double *__restrict__ h_new,
-    const uint8_t *__restrict__ rcv, const uint32_t *__restrict__ cnt,
-    const double *__restrict__ powtab, int W, int H, int tiles_x, ErodeConst k, int32_t *hdr) {
+    const uint8_t *__restrict__ rcv, const uint8_t *__restrict__ lvl,
+    const uint32_t *__restrict__ cnt, const double *__restrict__ powtab, int W, int H,
+    int tiles_x, int ntiles, ErodeConst k, int32_t *hdr) {
+  constexpr int kWaves = kEroBlock / kWave;
+  constexpr int kKeys = kHalo + 1;
+  constexpr int kPer = kRegCells / kEroBlock;
   __shared__ double hs[kRegCells];
   __shared__ uint8_t r_s[kRegCells];
   __shared__ uint8_t d_s[kRegCells];
   __shared__ uint16_t list[kRegCells];
-  __shared__ int lcount[kHalo + 2], loff[kHalo + 2];
-  __shared__ int s_maxl;
-  const int x0 = (blockIdx.x % tiles_x) * kTile, y0 = (blockIdx.x / tiles_x) * kTile;
+  __shared__ int wcount[kKeys][kWaves];
+  __shared__ int loff[kKeys + 1];
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
   const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
-  if (threadIdx.x < kHalo + 2) lcount[threadIdx.x] = 0;
-  if (threadIdx.x == 0) s_maxl = 0;
-  for (int i = threadIdx.x; i < kRegCells; i += blockDim.x) {
-    const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
-    const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
-    const size_t g = static_cast<size_t>(gy) * W + gx;
-    r_s[i] = in ? rcv[g] : kOutside;
-    hs[i] = in ? h_old[g] : 0.0;
-    d_s[i] = kEscaped;
+  const int wave = threadIdx.x / kWave;
+  for (int i = threadIdx.x; i < kKeys * kWaves; i += kEroBlock) (&wcount[0][0])[i] = 0;
+  load_region<kEroBlock, true>(rcv, h_old, lvl, r_s, hs, d_s, rx0, ry0, W, H);
+  __syncthreads();
+  // bucket the region cells by level: rank within (level, wave), then bases
+  int rank[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int d = d_s[q * kEroBlock + threadIdx.x];
+    rank[q] = d < kKeys ? atomicAdd(&wcount[d][wave], 1) : -1;
   }
   __syncthreads();
-  // mark the chains of the tile cells with their levels
-  int maxl = 0;
-  for (int i = threadIdx.x; i < kTile * kTile; i += blockDim.x) {
-    const int tx = i % kTile, ty = i / kTile;
-    if (x0 + tx >= W || y0 + ty >= H) continue;
-    int lx = tx + kHalo, ly = ty + kHalo, len = 0;
-    bool ok = true;
-    while (true) {
-      const int r = r_s[ly * kReg + lx];
-      if (r >= 8) break;
-      if (!region_step(r, lx, ly) || ++len > kHalo) {
-        ok = false;
-        break;
-      }
+  if (threadIdx.x < kKeys) {
+    int run = 0;
+    for (int w = 0; w < kWaves; w++) {
+      const int c = wcount[threadIdx.x][w];
+      wcount[threadIdx.x][w] = run;
+      run += c;
     }
-    if (!ok) {                 // cannot happen when pass 1 found every level <= kHalo
-      atomicOr(&hdr[2], kErrDepth);
-      continue;
-    }
-    maxl = max(maxl, len);
-    lx = tx + kHalo;
-    ly = ty + kHalo;
-    for (int d = len; d >= 0; d--) {
-      d_s[ly * kReg + lx] = static_cast<uint8_t>(d);
-      if (d > 0) region_step(r_s[ly * kReg + lx], lx, ly);
-    }
+    loff[threadIdx.x + 1] = run;
   }
-  atomicMax(&s_maxl, maxl);
-  __syncthreads();
-  for (int i = threadIdx.x; i < kRegCells; i += blockDim.x)
-    if (d_s[i] != kEscaped) atomicAdd(&lcount[d_s[i]], 1);
   __syncthreads();
   if (threadIdx.x == 0) {
-    int run = 0;
-    for (int L = 0; L <= s_maxl; L++) {
-      loff[L] = run;
-      run += lcount[L];
-      lcount[L] = loff[L];
-    }
-    loff[s_maxl + 1] = run;
+    loff[0] = 0;
+    for (int L = 1; L <= kKeys; L++) loff[L] += loff[L - 1];
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kRegCells; i += blockDim.x)
-    if (d_s[i] != kEscaped) list[atomicAdd(&lcount[d_s[i]], 1)] = static_cast<uint16_t>(i);
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int i = q * kEroBlock + threadIdx.x;
+    const int d = d_s[i];
+    if (rank[q] >= 0) list[loff[d] + wcount[d][wave] + rank[q]] = static_cast<uint16_t>(i);
+  }
   __syncthreads();
   // level 0: uplift of interior outlets; rings 0 and 1 stay fixed
-  for (int e = loff[0] + threadIdx.x; e < loff[1]; e += blockDim.x) {
+  for (int e = threadIdx.x; e < loff[1]; e += kEroBlock) {
     const int i = list[e];
     const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
     if (gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2) hs[i] += k.ud;
   }
   __syncthreads();
-  for (int L = 1; L <= s_maxl; L++) {
-    for (int e = loff[L] + threadIdx.x; e < loff[L + 1]; e += blockDim.x) {
+  for (int L = 1; L < kKeys; L++) {
+    if (loff[L] == loff[L + 1]) break;        // uniform: levels are contiguous from 0
+    for (int e = loff[L] + threadIdx.x; e < loff[L + 1]; e += kEroBlock) {
       const int i = list[e];
       const int r = r_s[i];
       int lx = i % kReg, ly = i / kReg;
       const size_t g = static_cast<size_t>(ry0 + ly) * W + (rx0 + lx);
-      region_step(r, lx, ly);
+      if (!region_step(r, lx, ly)) continue;  // receiver beyond the region: not needed
       const double fact = (k.kdt * powtab[cnt[g]]) / ((r & 1) ? k.lenpow1 : k.lenpow0);
       hs[i] = newton_update<NMODE>(hs[i] + k.ud, hs[ly * kReg + lx], fact, k, hdr);
     }
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < kTile * kTile; i += blockDim.x) {
+  for (int it = 0; it < kTile * kTile / kEroBlock; it++) {
+    const int i = it * kEroBlock + threadIdx.x;
     const int tx = i % kTile, ty = i / kTile;
     const int gx = x0 + tx, gy = y0 + ty;
     if (gx >= W || gy >= H) continue;
@@ -919,7 +885,7 @@ struct fs_model {
 
   // device state; h and h_alt swap after each tile-path step
   double *h = nullptr, *h_alt = nullptr;
-  uint8_t *rcv = nullptr, *dmask = nullptr;
+  uint8_t *rcv = nullptr, *dmask = nullptr, *lvl = nullptr;
   uint16_t *depth = nullptr;
   unsigned long long *jump = nullptr;
   int32_t *list = nullptr, *stack = nullptr, *hdr = nullptr, *levels_d = nullptr;
@@ -1030,7 +996,8 @@ void free_model(fs_model *m) {
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   for (void *ptr : {static_cast<void *>(m->h), static_cast<void *>(m->h_alt),
                     static_cast<void *>(m->rcv),
-                    static_cast<void *>(m->dmask), static_cast<void *>(m->depth),
+                    static_cast<void *>(m->dmask), static_cast<void *>(m->lvl),
+                    static_cast<void *>(m->depth),
                     static_cast<void *>(m->jump), static_cast<void *>(m->list),
                     static_cast<void *>(m->stack), static_cast<void *>(m->hdr),
                     static_cast<void *>(m->levels_d), static_cast<void *>(m->cnt),
@@ -1116,6 +1083,7 @@ int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_
   FS_TRY(dev_alloc(&m->h_alt, N));
   FS_TRY(dev_alloc(&m->rcv, N + 16));
   FS_TRY(dev_alloc(&m->dmask, N));
+  FS_TRY(dev_alloc(&m->lvl, N));
   FS_TRY(dev_alloc(&m->depth, N));
   FS_TRY(dev_alloc(&m->jump, N));
   FS_TRY(dev_alloc(&m->list, N));
@@ -1391,7 +1359,7 @@ int level_path(fs_model *m, hipEvent_t *ev) {
 template <int NMODE>
 int launch_tile_erode(fs_model *m, int tiles, int tiles_x) {
   hipLaunchKernelGGL(k_tile_erode<NMODE>, dim3(tiles), dim3(kEroBlock), 0, m->stream, m->h, m->h_alt,
-                     m->rcv, m->cnt, m->powtab, m->W, m->H, tiles_x, m->ek, m->hdr);
+                     m->rcv, m->lvl, m->cnt, m->powtab, m->W, m->H, tiles_x, tiles, m->ek, m->hdr);
   return check_launch();
 }
 
@@ -1403,24 +1371,20 @@ int tile_path(fs_model *m, hipEvent_t *ev, bool *done) {
   hipStream_t s = m->stream;
   const int tiles_x = (m->W + kTile - 1) / kTile, tiles_y = (m->H + kTile - 1) / kTile;
   const int tiles = tiles_x * tiles_y;
-  FS_TRY(launch_fill_counts(m));
+  hipLaunchKernelGGL(k_fill_counts, dim3(1), dim3(64), 0, s, m->hdr, 0);
+  FS_TRY(check_launch());
   FS_TRY(kt_begin(m, FS_K_TILE_ACC));
-  hipLaunchKernelGGL(k_tile_acc, dim3(tiles), dim3(kAccBlock), 0, s, m->rcv, m->cnt, m->hdr, m->W,
-                     m->H, tiles_x);
+  hipLaunchKernelGGL(k_tile_acc, dim3(tiles), dim3(kAccBlock), 0, s, m->rcv, m->cnt, m->lvl, m->hdr,
+                     m->W, m->H, tiles_x, tiles);
   FS_TRY(check_launch());
   FS_TRY(kt_end(m));
   FS_HIP_TRY(hipEventRecord(ev[2], s));
-  FS_TRY(fetch_header(m, kHdrCounts));
+  FS_TRY(fetch_header(m, 0));
   FS_TRY(stage_collect(m, m->ev_cur ^ 1));
   FS_TRY(kt_collect(m));
   FS_TRY(device_error_status(m->hdr_h[2]));
-  const int D = m->hdr_h[0];
-  if (m->hdr_h[3] != 0 || D > kHalo) {
-    m->prev_maxdepth = std::max(m->prev_maxdepth, std::min(D, FS_MAX_LEVELS));
-    return FS_OK;
-  }
-  FS_TRY(take_levels(m, D));
-  m->order_current = false;                 // stack / donor masks rebuilt on request
+  if (m->hdr_h[3] != 0) return FS_OK;       // some level exceeds the halo
+  m->order_current = false;                 // levels, stack, donor masks on request
   FS_HIP_TRY(hipEventRecord(ev[3], s));
   FS_HIP_TRY(hipEventRecord(ev[4], s));
   FS_HIP_TRY(hipEventRecord(ev[5], s));
@@ -1633,6 +1597,8 @@ int fs_get_stack(fs_model *m, int32_t *stack, size_t count) {
 
 int fs_get_levels(fs_model *m, int32_t *levels, size_t capacity, int32_t *nlevel) {
   if (!m || !nlevel) return FS_ERR_INVALID_ARGUMENT;
+  FS_TRY(check_model(m));
+  FS_TRY(ensure_order(m));
   *nlevel = m->have_step ? m->nlevel : 0;
   if (!levels) return FS_OK;
   if (capacity < static_cast<size_t>(*nlevel)) return FS_ERR_BUFFER_SIZE;

@@ -127,7 +127,7 @@ int fs_set_h(fs_model *m, const double *in, size_t count);
  * times are device time between HIP events bracketing each stage; Overall is
  * host wall time of fs_run / fs_step plus construction.  On the tile path
  * (below) donors, levels and accumulation are one kernel, reported under
- * DetermineDonors; GenerateOrder is the level-histogram read-back; uplift is
+ * DetermineDonors; GenerateOrder is the eligibility read-back; uplift is
  * fused into Erosion.  FS_PATH_LEVELS gives the reference's stage split. */
 int fs_get_timers(fs_model *m, uint64_t us[FS_NUM_TIMERS]);
 int fs_reset_timers(fs_model *m);
@@ -147,8 +147,9 @@ int fs_set_step_path(fs_model *m, int path);
 int fs_path_stats(fs_model *m, uint64_t *tile_steps, uint64_t *level_steps);
 
 /* ---- state of the last completed time-step (parity checks) -------------- */
-/* After a tile-path step the stack and donor lists are rebuilt from that
- * step's receivers on the first request (same kernels as the level path). */
+/* After a tile-path step, levels[], the stack and the donor lists are
+ * rebuilt from that step's receivers on the first request (the level path's
+ * kernels); accumulation and receivers are always kept. */
 /* rec[] in the reference encoding: 0..7 or FS_NO_FLOW (RB:86, 160-182). */
 int fs_get_receivers(fs_model *m, int32_t *rec, size_t count);
 /* ndon[count] and donor[8*count] as ComputeDonors leaves them
