@@ -42,6 +42,7 @@
 #include <vector>
 
 #include "fs_internal.h"
+#include "fs_receiver.h"
 
 namespace {
 
@@ -205,31 +206,37 @@ __global__ void k_fill_counts(int32_t *hdr, int n_counts) {
   if (i < n_counts) hdr[kHdrInts + i] = 0;
 }
 
-// ComputeReceivers (RB:160-182).  64x4 threads per block, one interior cell
-// each; neighbour rows are shared through L1/L2.  Directions are tried in
-// order 0..7 with a strict '>' from 0, so ties keep the lowest direction and
-// flat cells and pits stay NO_FLOW.
+// ComputeReceivers (RB:160-182).  Each thread owns one column of a strip of
+// kRecvRows interior rows and slides a 3x3 window down it, so every
+// elevation is fetched from HBM about once (neighbouring columns come from
+// L1).  fs::receiver_fast picks the reference's direction -- steepest
+// (hc - h[n]) / dr[n] over n = 0..7 with a strict '>' from 0, ties to the
+// lowest n, flat cells and pits NO_FLOW -- with two IEEE divisions instead of
+// eight (see fs_receiver.h).
+constexpr int kRecvRows = 16;
 __global__ __launch_bounds__(256) void k_receivers(const double *__restrict__ h,
-                                                   uint8_t *__restrict__ rcv, int W,
-                                                   int H, double d_card, double d_diag) {
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
+                                                   uint8_t *__restrict__ rcv, int W, int H,
+                                                   double d_card, double d_diag) {
+  const int x = blockIdx.x * 256 + threadIdx.x + 2;
   if (x >= W - 2) return;
-  for (int y = blockIdx.y * 4 + (threadIdx.x >> 6) + 2; y < H - 2; y += gridDim.y * 4) {
-    const size_t c = static_cast<size_t>(y) * W + x;
-    const double *up = h + c - W;
-    const double *dn = h + c + W;
-    const double hc = h[c];
-    double ms = 0.0, s;
-    int mn = kNoFlow;
-    s = (hc - h[c - 1]) / d_card; if (s > ms) { ms = s; mn = 0; }
-    s = (hc - up[-1]) / d_diag;   if (s > ms) { ms = s; mn = 1; }
-    s = (hc - up[0]) / d_card;    if (s > ms) { ms = s; mn = 2; }
-    s = (hc - up[1]) / d_diag;    if (s > ms) { ms = s; mn = 3; }
-    s = (hc - h[c + 1]) / d_card; if (s > ms) { ms = s; mn = 4; }
-    s = (hc - dn[1]) / d_diag;    if (s > ms) { ms = s; mn = 5; }
-    s = (hc - dn[0]) / d_card;    if (s > ms) { ms = s; mn = 6; }
-    s = (hc - dn[-1]) / d_diag;   if (s > ms) { ms = s; mn = 7; }
-    rcv[c] = static_cast<uint8_t>(mn);
+  const int nstrips = (H - 4 + kRecvRows - 1) / kRecvRows;
+  for (int strip = blockIdx.y; strip < nstrips; strip += gridDim.y) {
+    const int y0 = strip * kRecvRows + 2;
+    const int y1 = min(y0 + kRecvRows, H - 2);
+    const double *p = h + static_cast<size_t>(y0 - 1) * W + x;
+    double a0 = p[-1], a1 = p[0], a2 = p[1];        // row y-1
+    p += W;
+    double b0 = p[-1], b1 = p[0], b2 = p[1];        // row y
+    for (int y = y0; y < y1; y++) {
+      p += W;
+      const double c0 = p[-1], c1 = p[0], c2 = p[1];  // row y+1
+      // neighbours in direction order W, NW, N, NE, E, SE, S, SW (RB:79-82)
+      const double nb[8] = {b0, a0, a1, a2, b2, c2, c1, c0};
+      rcv[static_cast<size_t>(y) * W + x] =
+          static_cast<uint8_t>(fs::receiver_fast(b1, nb, d_card, d_diag));
+      a0 = b0; a1 = b1; a2 = b2;
+      b0 = c0; b1 = c1; b2 = c2;
+    }
   }
 }
 
@@ -587,7 +594,7 @@ __device__ __forceinline__ double newton_update(double h0, double hn, double fac
 template <int NMODE>
 __device__ __forceinline__ void erode_cell(int32_t c, const uint8_t *__restrict__ rcv,
                                            const uint32_t *__restrict__ cnt,
-                                           const double *__restrict__ powtab, double *h,
+                                           const double *__restrict__ ftab, double *h,
                                            int W, int32_t N, const ErodeConst &k,
                                            int32_t *hdr) {
   if (static_cast<uint32_t>(c) >= static_cast<uint32_t>(N)) {
@@ -602,32 +609,31 @@ __device__ __forceinline__ void erode_cell(int32_t c, const uint8_t *__restrict_
     atomicOr(&hdr[2], kErrIndex);
     a = static_cast<uint32_t>(N);
   }
-  const double fact = (k.kdt * powtab[a]) / ((r & 1) ? k.lenpow1 : k.lenpow0);
-  h[c] = newton_update<NMODE>(h[c] + k.ud, h[n], fact, k, hdr);
+  h[c] = newton_update<NMODE>(h[c] + k.ud, h[n], ftab[2 * a + (r & 1)], k, hdr);
 }
 
 template <int NMODE>
 __global__ __launch_bounds__(256) void k_erode(const int32_t *__restrict__ stack,
                                                const uint8_t *__restrict__ rcv,
                                                const uint32_t *__restrict__ cnt,
-                                               const double *__restrict__ powtab, double *h,
+                                               const double *__restrict__ ftab, double *h,
                                                int32_t ls, int32_t le, int W, int32_t N,
                                                ErodeConst k, int32_t *hdr) {
   const long long p = static_cast<long long>(ls) + blockIdx.x * 256ll + threadIdx.x;
   if (p >= le) return;
-  erode_cell<NMODE>(stack[p], rcv, cnt, powtab, h, W, N, k, hdr);
+  erode_cell<NMODE>(stack[p], rcv, cnt, ftab, h, W, N, k, hdr);
 }
 
 template <int NMODE>
 __global__ __launch_bounds__(kSmallBlock) void k_erode_small(
     const int32_t *__restrict__ stack, const uint8_t *__restrict__ rcv,
-    const uint32_t *__restrict__ cnt, const double *__restrict__ powtab, double *h,
+    const uint32_t *__restrict__ cnt, const double *__restrict__ ftab, double *h,
     const int32_t *__restrict__ levels, int L_lo, int L_hi, int W, int32_t N, ErodeConst k,
     int32_t *hdr) {
   for (int L = L_lo; L <= L_hi; L++) {
     const int32_t ls = levels[L], le = levels[L + 1];
     for (long long p = ls + threadIdx.x; p < le; p += kSmallBlock)
-      erode_cell<NMODE>(stack[p], rcv, cnt, powtab, h, W, N, k, hdr);
+      erode_cell<NMODE>(stack[p], rcv, cnt, ftab, h, W, N, k, hdr);
     __syncthreads();
   }
 }
@@ -649,6 +655,7 @@ constexpr int kAccBlock = 512;
 constexpr int kEroBlock = 1024;
 constexpr uint8_t kOutside = 0xFD;            // region cell beyond the grid
 constexpr uint8_t kEscaped = 0xFF;            // chain leaves the region
+constexpr int kFactCache = 256;               // erosion factors cached in LDS
 
 // Region-local neighbour step in direction r; false if it leaves the region.
 __device__ __forceinline__ bool region_step(int r, int &lx, int &ly) {
@@ -674,13 +681,15 @@ __device__ __forceinline__ int xcd_tile(int b, int ntiles) {
 template <int BLOCK, bool WITH_H>
 __device__ __forceinline__ void load_region(const uint8_t *__restrict__ rcv,
                                             const double *__restrict__ h,
-                                            const uint8_t *__restrict__ lvl, uint8_t *r_s,
-                                            double *hs, uint8_t *d_s, int rx0, int ry0, int W,
-                                            int H) {
+                                            const uint8_t *__restrict__ lvl,
+                                            const uint32_t *__restrict__ cnt, uint8_t *r_s,
+                                            double *hs, uint8_t *d_s, uint32_t *c_s, int rx0,
+                                            int ry0, int W, int H) {
   constexpr int kPer = kRegCells / BLOCK;
   static_assert(kRegCells % BLOCK == 0, "region must split evenly over the block");
   uint8_t rv[kPer], dv[WITH_H ? kPer : 1];
   double hv[WITH_H ? kPer : 1];
+  uint32_t cv[WITH_H ? kPer : 1];
 #pragma unroll
   for (int k = 0; k < kPer; k++) {
     const int i = k * BLOCK + threadIdx.x;
@@ -691,6 +700,7 @@ __device__ __forceinline__ void load_region(const uint8_t *__restrict__ rcv,
     if (WITH_H) {
       hv[k] = in ? h[g] : 0.0;
       dv[k] = in ? lvl[g] : kEscaped;
+      cv[k] = in ? cnt[g] : 0u;
     }
   }
 #pragma unroll
@@ -700,6 +710,7 @@ __device__ __forceinline__ void load_region(const uint8_t *__restrict__ rcv,
     if (WITH_H) {
       hs[i] = hv[k];
       d_s[i] = dv[k];
+      c_s[i] = cv[k];
     }
   }
 }
@@ -723,29 +734,50 @@ __global__ __launch_bounds__(kAccBlock) void k_tile_acc(const uint8_t *__restric
   const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
   if (threadIdx.x == 0) s_over = 0;
   for (int it = 0; it < kRegCells / kAccBlock; it++) c_s[it * kAccBlock + threadIdx.x] = 0;
-  load_region<kAccBlock, false>(rcv, nullptr, nullptr, r_s, nullptr, nullptr, rx0, ry0, W, H);
+  load_region<kAccBlock, false>(rcv, nullptr, nullptr, nullptr, r_s, nullptr, nullptr, nullptr,
+                                rx0, ry0, W, H);
   __syncthreads();
+  // kWalkers chains per thread advance together so that their dependent LDS
+  // reads overlap; a walker stops at an outlet, at the region edge, or
+  // after kHalo hops.
+  constexpr int kPer = kRegCells / kAccBlock;
+  constexpr int kWalkers = 6;
+  static_assert(kPer % kWalkers == 0, "walkers must split the thread's cells evenly");
   int over = 0;
-  for (int it = 0; it < kRegCells / kAccBlock; it++) {
-    const int i = it * kAccBlock + threadIdx.x;
-    if (r_s[i] == kOutside) continue;
-    int lx = i % kReg, ly = i / kReg, hops = 0;
-    bool rooted = false;
-    while (true) {
-      const int x = ly * kReg + lx;
-      atomicAdd(&c_s[x], 1u);
-      const int r = r_s[x];
-      if (r >= 8) {
-        rooted = true;
-        break;
+  for (int g = 0; g < kPer; g += kWalkers) {
+    int wx[kWalkers], wy[kWalkers], hops[kWalkers];
+    uint32_t live = 0, rooted = 0;
+#pragma unroll
+    for (int j = 0; j < kWalkers; j++) {
+      const int i = (g + j) * kAccBlock + threadIdx.x;
+      wx[j] = i % kReg;
+      wy[j] = i / kReg;
+      hops[j] = 0;
+      if (r_s[i] != kOutside) live |= 1u << j;
+    }
+    while (live) {
+#pragma unroll
+      for (int j = 0; j < kWalkers; j++) {
+        if (!(live & (1u << j))) continue;
+        const int x = wy[j] * kReg + wx[j];
+        atomicAdd(&c_s[x], 1u);
+        const int r = r_s[x];
+        if (r >= 8) {
+          rooted |= 1u << j;
+          live &= ~(1u << j);
+        } else if (!region_step(r, wx[j], wy[j]) || ++hops[j] > kHalo) {
+          live &= ~(1u << j);
+        }
       }
-      if (!region_step(r, lx, ly) || ++hops > kHalo) break;
     }
     // the level of a tile cell is the length of its walk
-    const int tx = i % kReg - kHalo, ty = i / kReg - kHalo;
-    if (tx >= 0 && tx < kTile && ty >= 0 && ty < kTile) {
-      const size_t g = static_cast<size_t>(y0 + ty) * W + (x0 + tx);
-      if (rooted) lvl[g] = static_cast<uint8_t>(hops);
+#pragma unroll
+    for (int j = 0; j < kWalkers; j++) {
+      const int i = (g + j) * kAccBlock + threadIdx.x;
+      const int tx = i % kReg - kHalo, ty = i / kReg - kHalo;
+      if (r_s[i] == kOutside || tx < 0 || tx >= kTile || ty < 0 || ty >= kTile) continue;
+      const size_t gi = static_cast<size_t>(y0 + ty) * W + (x0 + tx);
+      if (rooted & (1u << j)) lvl[gi] = static_cast<uint8_t>(hops[j]);
       else over = 1;
     }
   }
@@ -762,32 +794,43 @@ __global__ __launch_bounds__(kAccBlock) void k_tile_acc(const uint8_t *__restric
 }
 
 // Pass 2: uplift and erosion (AddUplift RB:285-296, Erode RB:306-328) of the
-// tile's cells.  Levels come from pass 1; every region cell is processed
-// level by level from the outlets up with the reference's Newton iteration
-// on LDS-resident elevations (the tile cells' receiver chains lie in the
-// region because every level is at most kHalo).  Reads h_old, writes h_new:
+// tile's cells.  Levels come from pass 1.  Region cells are bucketed by
+// level into a list whose level segments are padded to whole 64-entry
+// chunks; each wave takes chunks in increasing order and each lane solves
+// its cell with the reference's Newton iteration on LDS-resident elevations
+// as soon as its receiver (one level lower, hence in an earlier chunk) is
+// marked done.  With one level per chunk no lane waits on a lane of its own
+// wave, and the lowest chunk in progress never waits on an unfinished one,
+// so the wavefront always advances; the wait is bounded all the same and
+// reported in hdr[2].  The tile cells' receiver chains lie in the region
+// because every level is at most kHalo.  Reads h_old, writes h_new:
 // neighbouring tiles read this tile's cells as their halo.
+constexpr uint8_t kDone = 0xFD;
+constexpr int kListCap = kRegCells + (kHalo + 1) * (kWave - 1);
 template <int NMODE>
 __global__ __launch_bounds__(kEroBlock) void k_tile_erode(
     const double *__restrict__ h_old, double *__restrict__ h_new,
     const uint8_t *__restrict__ rcv, const uint8_t *__restrict__ lvl,
-    const uint32_t *__restrict__ cnt, const double *__restrict__ powtab, int W, int H,
+    const uint32_t *__restrict__ cnt, const double *__restrict__ ftab, int W, int H,
     int tiles_x, int ntiles, ErodeConst k, int32_t *hdr) {
   constexpr int kWaves = kEroBlock / kWave;
   constexpr int kKeys = kHalo + 1;
   constexpr int kPer = kRegCells / kEroBlock;
   __shared__ double hs[kRegCells];
   __shared__ uint8_t r_s[kRegCells];
-  __shared__ uint8_t d_s[kRegCells];
-  __shared__ uint16_t list[kRegCells];
+  __shared__ uint8_t d_s[kRegCells];       // level, then kDone once solved
+  __shared__ uint16_t list[kListCap];
+  __shared__ uint32_t c_s[kRegCells];
+  __shared__ double ft_s[2 * kFactCache];
   __shared__ int wcount[kKeys][kWaves];
-  __shared__ int loff[kKeys + 1];
+  __shared__ int ltotal[kKeys], loff[kKeys + 1];
   const int t = xcd_tile(blockIdx.x, ntiles);
   const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
   const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
-  const int wave = threadIdx.x / kWave;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   for (int i = threadIdx.x; i < kKeys * kWaves; i += kEroBlock) (&wcount[0][0])[i] = 0;
-  load_region<kEroBlock, true>(rcv, h_old, lvl, r_s, hs, d_s, rx0, ry0, W, H);
+  if (threadIdx.x < 2 * kFactCache) ft_s[threadIdx.x] = ftab[threadIdx.x];
+  load_region<kEroBlock, true>(rcv, h_old, lvl, cnt, r_s, hs, d_s, c_s, rx0, ry0, W, H);
   __syncthreads();
   // bucket the region cells by level: rank within (level, wave), then bases
   int rank[kPer];
@@ -804,12 +847,16 @@ __global__ __launch_bounds__(kEroBlock) void k_tile_erode(
       wcount[threadIdx.x][w] = run;
       run += c;
     }
-    loff[threadIdx.x + 1] = run;
+    ltotal[threadIdx.x] = run;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    loff[0] = 0;
-    for (int L = 1; L <= kKeys; L++) loff[L] += loff[L - 1];
+    int run = 0;
+    for (int L = 0; L < kKeys; L++) {
+      loff[L] = run;
+      run += (ltotal[L] + kWave - 1) / kWave * kWave;   // whole chunks per level
+    }
+    loff[kKeys] = run;
   }
   __syncthreads();
 #pragma unroll
@@ -818,27 +865,41 @@ __global__ __launch_bounds__(kEroBlock) void k_tile_erode(
     const int d = d_s[i];
     if (rank[q] >= 0) list[loff[d] + wcount[d][wave] + rank[q]] = static_cast<uint16_t>(i);
   }
+  if (threadIdx.x < kKeys)
+    for (int e = loff[threadIdx.x] + ltotal[threadIdx.x]; e < loff[threadIdx.x + 1]; e++)
+      list[e] = 0xFFFF;
   __syncthreads();
-  // level 0: uplift of interior outlets; rings 0 and 1 stay fixed
-  for (int e = threadIdx.x; e < loff[1]; e += kEroBlock) {
-    const int i = list[e];
-    const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
-    if (gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2) hs[i] += k.ud;
-  }
-  __syncthreads();
-  for (int L = 1; L < kKeys; L++) {
-    if (loff[L] == loff[L + 1]) break;        // uniform: levels are contiguous from 0
-    for (int e = loff[L] + threadIdx.x; e < loff[L + 1]; e += kEroBlock) {
-      const int i = list[e];
+  const int nchunks = loff[kKeys] / kWave, root_chunks = loff[1] / kWave;
+  for (int ch = wave; ch < nchunks; ch += kWaves) {
+    const int i = list[ch * kWave + lane];
+    if (i == 0xFFFF) continue;
+    if (ch < root_chunks) {
+      // level 0: uplift of interior outlets; rings 0 and 1 stay fixed
+      const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
+      if (gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2) hs[i] += k.ud;
+    } else {
       const int r = r_s[i];
       int lx = i % kReg, ly = i / kReg;
-      const size_t g = static_cast<size_t>(ry0 + ly) * W + (rx0 + lx);
-      if (!region_step(r, lx, ly)) continue;  // receiver beyond the region: not needed
-      const double fact = (k.kdt * powtab[cnt[g]]) / ((r & 1) ? k.lenpow1 : k.lenpow0);
-      hs[i] = newton_update<NMODE>(hs[i] + k.ud, hs[ly * kReg + lx], fact, k, hdr);
+      if (region_step(r, lx, ly)) {          // else the receiver is not needed here
+        const int nb = ly * kReg + lx;
+        int spins = 0;
+        while (__hip_atomic_load(&d_s[nb], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) !=
+               kDone) {
+          if (++spins > (1 << 22)) {
+            atomicOr(&hdr[2], kErrLookback);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        const uint32_t a = c_s[i];
+        const int ti = 2 * static_cast<int>(a) + (r & 1);
+        const double fact = a < kFactCache ? ft_s[ti] : ftab[ti];
+        hs[i] = newton_update<NMODE>(hs[i] + k.ud, hs[nb], fact, k, hdr);
+      }
     }
-    __syncthreads();
+    __hip_atomic_store(&d_s[i], kDone, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
+  __syncthreads();
   for (int it = 0; it < kTile * kTile / kEroBlock; it++) {
     const int i = it * kEroBlock + threadIdx.x;
     const int tx = i % kTile, ty = i / kTile;
@@ -890,7 +951,7 @@ struct fs_model {
   unsigned long long *jump = nullptr;
   int32_t *list = nullptr, *stack = nullptr, *hdr = nullptr, *levels_d = nullptr;
   uint32_t *cnt = nullptr;
-  double *powtab = nullptr;
+  double *ftab = nullptr;   // erosion factor per (upstream count, diagonal)
   uint64_t *status = nullptr;
   unsigned long long *ticket = nullptr;
 
@@ -1001,7 +1062,7 @@ void free_model(fs_model *m) {
                     static_cast<void *>(m->jump), static_cast<void *>(m->list),
                     static_cast<void *>(m->stack), static_cast<void *>(m->hdr),
                     static_cast<void *>(m->levels_d), static_cast<void *>(m->cnt),
-                    static_cast<void *>(m->powtab), static_cast<void *>(m->status),
+                    static_cast<void *>(m->ftab), static_cast<void *>(m->status),
                     static_cast<void *>(m->ticket)})
     if (ptr) (void)hipFree(ptr);
   if (m->hdr_h) (void)hipHostFree(m->hdr_h);
@@ -1089,7 +1150,9 @@ int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_
   FS_TRY(dev_alloc(&m->list, N));
   FS_TRY(dev_alloc(&m->stack, N));
   FS_TRY(dev_alloc(&m->cnt, N));
-  FS_TRY(dev_alloc(&m->powtab, N + 1));
+  // at least kFactCache counts, so the tile kernels' LDS cache stays in bounds
+  const size_t kmax = std::max<size_t>(N, kFactCache - 1);
+  FS_TRY(dev_alloc(&m->ftab, 2 * (kmax + 1)));
   FS_TRY(dev_alloc(&m->hdr, kHdrInts + FS_MAX_LEVELS + 3));
   FS_TRY(dev_alloc(&m->levels_d, FS_MAX_LEVELS + 4));
   FS_TRY(dev_alloc(&m->status, ntiles));
@@ -1109,12 +1172,13 @@ int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_
   {
     std::vector<double> tab;
     try {
-      tab.resize(N + 1);
+      tab.resize(2 * (kmax + 1));
     } catch (const std::bad_alloc &) {
       return FS_ERR_OUT_OF_MEMORY;
     }
-    fs::fill_pow_table(tab.data(), N, p->cell_area, p->meq);
-    FS_HIP_TRY(hipMemcpy(m->powtab, tab.data(), (N + 1) * sizeof(double), hipMemcpyHostToDevice));
+    fs::fill_fact_table(tab.data(), kmax, p->cell_area, p->meq, m->ek.kdt, m->ek.lenpow0,
+                        m->ek.lenpow1);
+    FS_HIP_TRY(hipMemcpy(m->ftab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   if (h_init) {
     FS_HIP_TRY(hipMemcpy(m->h, h_init, N * sizeof(double), hipMemcpyHostToDevice));
@@ -1180,7 +1244,7 @@ int launch_erosion(fs_model *m, const std::vector<Span> &spans) {
     if (s.small) {
       FS_TRY(kt_begin(m, FS_K_EROSION));
       hipLaunchKernelGGL(k_erode_small<NMODE>, dim3(1), dim3(kSmallBlock), 0, m->stream, m->stack,
-                         m->rcv, m->cnt, m->powtab, m->h, m->levels_d, s.lo, s.hi, m->W, m->N,
+                         m->rcv, m->cnt, m->ftab, m->h, m->levels_d, s.lo, s.hi, m->W, m->N,
                          m->ek, hdr);
       FS_TRY(check_launch());
       FS_TRY(kt_end(m));
@@ -1189,7 +1253,7 @@ int launch_erosion(fs_model *m, const std::vector<Span> &spans) {
       const unsigned blocks = static_cast<unsigned>((le - ls + 255) / 256);
       FS_TRY(kt_begin(m, FS_K_EROSION));
       hipLaunchKernelGGL(k_erode<NMODE>, dim3(blocks), dim3(256), 0, m->stream, m->stack, m->rcv,
-                         m->cnt, m->powtab, m->h, ls, le, m->W, m->N, m->ek, hdr);
+                         m->cnt, m->ftab, m->h, ls, le, m->W, m->N, m->ek, hdr);
       FS_TRY(check_launch());
       FS_TRY(kt_end(m));
     }
@@ -1198,8 +1262,8 @@ int launch_erosion(fs_model *m, const std::vector<Span> &spans) {
 }
 
 int launch_receivers(fs_model *m) {
-  const unsigned gx = static_cast<unsigned>((m->W - 4 + 63) / 64);
-  const unsigned gy = static_cast<unsigned>(std::min((m->H - 4 + 3) / 4, 65535));
+  const unsigned gx = static_cast<unsigned>((m->W - 4 + 255) / 256);
+  const unsigned gy = static_cast<unsigned>(std::min((m->H - 4 + kRecvRows - 1) / kRecvRows, 65535));
   FS_TRY(kt_begin(m, FS_K_RECEIVERS));
   hipLaunchKernelGGL(k_receivers, dim3(gx, gy), dim3(256), 0, m->stream, m->h, m->rcv, m->W, m->H,
                      m->d_card, m->d_diag);
@@ -1359,7 +1423,7 @@ int level_path(fs_model *m, hipEvent_t *ev) {
 template <int NMODE>
 int launch_tile_erode(fs_model *m, int tiles, int tiles_x) {
   hipLaunchKernelGGL(k_tile_erode<NMODE>, dim3(tiles), dim3(kEroBlock), 0, m->stream, m->h, m->h_alt,
-                     m->rcv, m->lvl, m->cnt, m->powtab, m->W, m->H, tiles_x, tiles, m->ek, m->hdr);
+                     m->rcv, m->lvl, m->cnt, m->ftab, m->W, m->H, tiles_x, tiles, m->ek, m->hdr);
   return check_launch();
 }
 

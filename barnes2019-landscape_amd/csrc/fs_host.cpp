@@ -72,11 +72,14 @@ void generate_terrain(double *h, int W, int H, uint32_t engine_seed) {
     }
 }
 
-// Table of pow(k*cell_area, m) for k = 0..kmax, evaluated with the host libm
-// exactly as the reference evaluates std::pow(accum[c], meq) (RB:315): accum
-// is always an exact integer multiple of cell_area, so indexing by the
-// upstream cell count k reproduces every value the reference can form.
-void fill_pow_table(double *tab, size_t kmax, double cell_area, double meq) {
+// Table of the erosion factor fact = keq*dt*pow(accum,meq)/pow(dr,neq)
+// (RB:315) for every value accum can take: accum is always k*cell_area for an
+// upstream cell count k, so tab[2k] (cardinal receiver, dr = dx) and
+// tab[2k+1] (diagonal, dr = dx*SQRT2) hold the factor evaluated with the host
+// libm and the reference's order of operations, ((keq*dt)*A^m)/dr^n, for
+// k = 0..kmax.
+void fill_fact_table(double *tab, size_t kmax, double cell_area, double meq, double kdt,
+                     double lenpow_card, double lenpow_diag) {
   unsigned nt = std::thread::hardware_concurrency();
   nt = std::max(1u, std::min(nt, 16u));
   const size_t n = kmax + 1;
@@ -87,8 +90,11 @@ void fill_pow_table(double *tab, size_t kmax, double cell_area, double meq) {
     const size_t a = t * chunk, b = std::min(n, a + chunk);
     if (a >= b) break;
     pool.emplace_back([=]() {
-      for (size_t k = a; k < b; k++)
-        tab[k] = std::pow(static_cast<double>(k) * cell_area, meq);
+      for (size_t k = a; k < b; k++) {
+        const double am = std::pow(static_cast<double>(k) * cell_area, meq);
+        tab[2 * k] = (kdt * am) / lenpow_card;
+        tab[2 * k + 1] = (kdt * am) / lenpow_diag;
+      }
     });
   }
   for (auto &th : pool) th.join();

@@ -14,7 +14,8 @@ namespace fs {
 void set_last_error(const std::string &msg);
 const char *last_error();
 void generate_terrain(double *h, int W, int H, uint32_t engine_seed);
-void fill_pow_table(double *tab, size_t kmax, double cell_area, double meq);
+void fill_fact_table(double *tab, size_t kmax, double cell_area, double meq, double kdt,
+                     double lenpow_card, double lenpow_diag);
 int write_dem(const char *filename, const double *h, int W, int H);
 const char *status_string(int s);
 
