@@ -42,893 +42,13 @@
 #include <vector>
 
 #include "fs_internal.h"
-#include "fs_receiver.h"
+#include "fs_hostutil.h"
 
-namespace {
-
-constexpr int kWave = 64;
-constexpr uint8_t kNoFlow = 0xFF;
-constexpr int kWalkCap = 64;           // hops walked per cell in k_topology
-constexpr int kJumpPasses = 12;        // kWalkCap * 2^10 > FS_MAX_LEVELS
-constexpr int kScanBlock = 256;
-constexpr int kRootItems = 16;         // cells per thread in k_roots
-constexpr int kRootTile = kScanBlock * kRootItems;
-constexpr int kBfsItems = 8;           // stack entries per thread in k_bfs
-constexpr int kBfsTile = kScanBlock * kBfsItems;
-constexpr int kSmallBlock = 1024;
-constexpr int kSmallItems = 4;
-constexpr int kSmallLevel = kSmallBlock * kSmallItems;  // one-workgroup levels
-constexpr int kNewtonCap = 100;
-constexpr int kHdrInts = 4;            // maxdepth, n_unfinished, err, spare
-constexpr int kHdrCounts = 1024;       // level counts fetched with the header
-constexpr int kTopoBlocks = 2048;
-constexpr int kJumpBlocks = 1024;
-
-// device error bits (hdr[2])
-constexpr int kErrLookback = 1, kErrNewton = 2, kErrIndex = 4, kErrDepth = 8;
-
-// ---------------------------------------------------------------------------
-// neighbourhood encodings
-// ---------------------------------------------------------------------------
-// Receiver direction r = 0..7 is W, NW, N, NE, E, SE, S, SW (RB:79-82); its
-// offset is nshift[r] = {-1, -W-1, -W, -W+1, 1, W+1, W, W-1} (RB:137).
-__device__ __forceinline__ int dir_offset(int r, int W) {
-  constexpr unsigned kDx = (0u << 0) | (0u << 2) | (1u << 4) | (2u << 6) |
-                           (2u << 8) | (2u << 10) | (1u << 12) | (0u << 14);
-  constexpr unsigned kDy = (1u << 0) | (0u << 2) | (0u << 4) | (0u << 6) |
-                           (1u << 8) | (2u << 10) | (2u << 12) | (2u << 14);
-  const int dx = static_cast<int>((kDx >> (2 * r)) & 3u) - 1;
-  const int dy = static_cast<int>((kDy >> (2 * r)) & 3u) - 1;
-  return dy * W + dx;
-}
-
-// Donor bitmask slot j = 0..7 is the neighbour at offset
-// {-W-1, -W, -W+1, -1, +1, W-1, W, W+1}[j]: ascending cell index, so the set
-// bits in slot order list the donors in the order RB's push builds them
-// (RB:194-202).  The neighbour in slot j is a donor iff its receiver
-// direction is kSlotDir[j].
-__device__ __forceinline__ int slot_offset(int j, int W) {
-  constexpr unsigned kSx = (0u << 0) | (1u << 2) | (2u << 4) | (0u << 6) |
-                           (2u << 8) | (0u << 10) | (1u << 12) | (2u << 14);
-  constexpr unsigned kSy = (0u << 0) | (0u << 2) | (0u << 4) | (1u << 6) |
-                           (1u << 8) | (2u << 10) | (2u << 12) | (2u << 14);
-  const int dx = static_cast<int>((kSx >> (2 * j)) & 3u) - 1;
-  const int dy = static_cast<int>((kSy >> (2 * j)) & 3u) - 1;
-  return dy * W + dx;
-}
-
-// ---------------------------------------------------------------------------
-// scans
-// ---------------------------------------------------------------------------
-// Exclusive prefix sum over the block; *total gets the block sum.  sh holds
-// BLOCK/64 + 1 words.  Ends with a barrier, so sh may be reused at once.
-template <int BLOCK>
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total,
-                                                    uint32_t *sh) {
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  uint32_t x = v;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const uint32_t y = __shfl_up(x, d);
-    if (lane >= d) x += y;
-  }
-  if (lane == kWave - 1) sh[wid] = x;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (int w = 0; w < BLOCK / kWave; w++) {
-      const uint32_t t = sh[w];
-      sh[w] = run;
-      run += t;
-    }
-    sh[BLOCK / kWave] = run;
-  }
-  __syncthreads();
-  const uint32_t ex = sh[wid] + x - v;
-  *total = sh[BLOCK / kWave];
-  __syncthreads();
-  return ex;
-}
-
-// Tile status word: [epoch:30 | flag:2 | value:32].  flag 1 = tile aggregate,
-// 2 = inclusive prefix.  A word whose epoch is not the launch's epoch is
-// "not yet published", so the array never needs clearing.
-__device__ __forceinline__ uint64_t status_word(uint32_t epoch, uint32_t flag, uint32_t v) {
-  return (static_cast<uint64_t>(epoch & 0x3fffffffu) << 34) |
-         (static_cast<uint64_t>(flag) << 32) | v;
-}
-
-// Decoupled look-back (Merrill & Garland, "Single-pass Parallel Prefix Scan
-// with Decoupled Look-back", 2016), run by one full wave.  Publishes the
-// tile's aggregate, sums predecessors' aggregates 64 at a time until an
-// inclusive prefix is found, publishes its own inclusive prefix and returns
-// the tile's exclusive prefix in every lane.  Tiles are numbered by an atomic
-// ticket, so every predecessor is resident and publishes its aggregate
-// without waiting; the wait is still bounded and reported in *err.
-__device__ uint32_t lookback(uint64_t *status, uint32_t tile, uint32_t epoch,
-                             uint32_t agg, int32_t *err) {
-  const int lane = threadIdx.x & (kWave - 1);
-  epoch &= 0x3fffffffu;
-  if (tile == 0) {
-    if (lane == 0)
-      __hip_atomic_store(&status[0], status_word(epoch, 2, agg), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    return 0;
-  }
-  if (lane == 0)
-    __hip_atomic_store(&status[tile], status_word(epoch, 1, agg), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  uint32_t excl = 0;
-  long long base = static_cast<long long>(tile) - 1;
-  int spins = 0;
-  while (true) {
-    const long long idx = base - lane;
-    uint64_t w = status_word(epoch, 2, 0);
-    if (idx >= 0)
-      w = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t flag = static_cast<uint32_t>(w >> 32) & 3u;
-    const bool valid = (static_cast<uint32_t>(w >> 34) == epoch) && flag != 0;
-    const unsigned long long inval = __ballot(!valid);
-    const unsigned long long pmask = __ballot(valid && flag == 2);
-    const int first_p = pmask ? __builtin_ctzll(pmask) : kWave;
-    const int first_inv = inval ? __builtin_ctzll(inval) : kWave;
-    if (first_inv < first_p) {
-      if (++spins > (1 << 22)) {
-        if (lane == 0) atomicOr(err, kErrLookback);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-      continue;
-    }
-    uint32_t v = (lane <= first_p) ? static_cast<uint32_t>(w) : 0u;
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    excl += v;
-    if (first_p < kWave) break;
-    base -= kWave;
-  }
-  if (lane == 0)
-    __hip_atomic_store(&status[tile], status_word(epoch, 2, excl + agg),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return excl;
-}
-
-// ---------------------------------------------------------------------------
-// kernels
-// ---------------------------------------------------------------------------
-__global__ void k_fill_counts(int32_t *hdr, int n_counts) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) {
-    hdr[0] = 0;   // maxdepth
-    hdr[1] = 0;   // n_unfinished
-    hdr[3] = 0;   // tiles whose levels exceed the tile halo
-  }
-  if (i < n_counts) hdr[kHdrInts + i] = 0;
-}
-
-// ComputeReceivers (RB:160-182).  Each thread owns one column of a strip of
-// kRecvRows interior rows and slides a 3x3 window down it, so every
-// elevation is fetched from HBM about once (neighbouring columns come from
-// L1).  fs::receiver_fast picks the reference's direction -- steepest
-// (hc - h[n]) / dr[n] over n = 0..7 with a strict '>' from 0, ties to the
-// lowest n, flat cells and pits NO_FLOW -- with two IEEE divisions instead of
-// eight (see fs_receiver.h).
-constexpr int kRecvRows = 16;
-__global__ __launch_bounds__(256) void k_receivers(const double *__restrict__ h,
-                                                   uint8_t *__restrict__ rcv, int W, int H,
-                                                   double d_card, double d_diag) {
-  const int x = blockIdx.x * 256 + threadIdx.x + 2;
-  if (x >= W - 2) return;
-  const int nstrips = (H - 4 + kRecvRows - 1) / kRecvRows;
-  for (int strip = blockIdx.y; strip < nstrips; strip += gridDim.y) {
-    const int y0 = strip * kRecvRows + 2;
-    const int y1 = min(y0 + kRecvRows, H - 2);
-    const double *p = h + static_cast<size_t>(y0 - 1) * W + x;
-    double a0 = p[-1], a1 = p[0], a2 = p[1];        // row y-1
-    p += W;
-    double b0 = p[-1], b1 = p[0], b2 = p[1];        // row y
-    for (int y = y0; y < y1; y++) {
-      p += W;
-      const double c0 = p[-1], c1 = p[0], c2 = p[1];  // row y+1
-      // neighbours in direction order W, NW, N, NE, E, SE, S, SW (RB:79-82)
-      const double nb[8] = {b0, a0, a1, a2, b2, c2, c1, c0};
-      rcv[static_cast<size_t>(y) * W + x] =
-          static_cast<uint8_t>(fs::receiver_fast(b1, nb, d_card, d_diag));
-      a0 = b0; a1 = b1; a2 = b2;
-      b0 = c0; b1 = c1; b2 = c2;
-    }
-  }
-}
-
-// Donor bitmask and level of every cell.  Rows are strided over the grid so
-// each block folds its level histogram in LDS before touching HBM.  The
-// level is found by walking receivers (all reads hit L1/L2: the receiver
-// field is one byte per cell and the walk moves one cell per hop); walks
-// longer than kWalkCap hops record (cell reached, hops) for k_jump.
-__global__ __launch_bounds__(256) void k_topology(
-    const uint8_t *__restrict__ rcv, uint8_t *__restrict__ dmask,
-    uint16_t *__restrict__ depth, unsigned long long *__restrict__ jump,
-    int32_t *__restrict__ list, int32_t *__restrict__ hdr, int W, int H) {
-  __shared__ int hist[kWalkCap + 1];
-  __shared__ int smax;
-  for (int i = threadIdx.x; i <= kWalkCap; i += blockDim.x) hist[i] = 0;
-  if (threadIdx.x == 0) smax = 0;
-  __syncthreads();
-  int32_t *counts = hdr + kHdrInts;
-  int local_max = 0;
-  for (int y = blockIdx.x; y < H; y += gridDim.x) {
-    const bool yin = y > 0 && y < H - 1;
-    for (int x = threadIdx.x; x < W; x += blockDim.x) {
-      const int c = y * W + x;
-      uint32_t m = 0;
-      if (yin && x > 0 && x < W - 1) {
-        const uint8_t *up = rcv + c - W;
-        const uint8_t *dn = rcv + c + W;
-        m |= (up[-1] == 5) ? 0x01u : 0u;
-        m |= (up[0] == 6) ? 0x02u : 0u;
-        m |= (up[1] == 7) ? 0x04u : 0u;
-        m |= (rcv[c - 1] == 4) ? 0x08u : 0u;
-        m |= (rcv[c + 1] == 0) ? 0x10u : 0u;
-        m |= (dn[-1] == 3) ? 0x20u : 0u;
-        m |= (dn[0] == 2) ? 0x40u : 0u;
-        m |= (dn[1] == 1) ? 0x80u : 0u;
-      }
-      dmask[c] = static_cast<uint8_t>(m);
-      int cur = c, d = 0;
-      int r = rcv[c];
-      while (r < 8 && d < kWalkCap) {
-        cur += dir_offset(r, W);
-        d++;
-        r = rcv[cur];
-      }
-      if (r >= 8) {
-        depth[c] = static_cast<uint16_t>(d);
-        atomicAdd(&hist[d], 1);
-        local_max = max(local_max, d);
-      } else {
-        depth[c] = 0xFFFF;
-        jump[c] = (static_cast<unsigned long long>(d) << 32) | static_cast<uint32_t>(cur);
-        const int slot = atomicAdd(&hdr[1], 1);
-        list[slot] = c;
-      }
-    }
-  }
-  atomicMax(&smax, local_max);
-  __syncthreads();
-  for (int i = threadIdx.x; i <= kWalkCap; i += blockDim.x)
-    if (hist[i]) atomicAdd(&counts[i], hist[i]);
-  if (threadIdx.x == 0) atomicMax(&hdr[0], smax);
-}
-
-// One pointer-jumping pass over the cells k_topology left unfinished: a cell
-// whose recorded ancestor has a level takes level = hops + that level;
-// otherwise it jumps to its ancestor's ancestor.  (ancestor, hops) pairs are
-// read and written as one 64-bit word, so every pair a thread sees is
-// consistent and the update is safe in place.
-__global__ __launch_bounds__(256) void k_jump(const int32_t *__restrict__ list,
-                                              uint16_t *depth, unsigned long long *jump,
-                                              int32_t *hdr) {
-  const int n = hdr[1];
-  int32_t *counts = hdr + kHdrInts;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int c = list[i];
-    if (__hip_atomic_load(&depth[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0xFFFF)
-      continue;
-    const unsigned long long j =
-        __hip_atomic_load(&jump[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int a = static_cast<int>(j & 0xffffffffu);
-    const unsigned d = static_cast<unsigned>(j >> 32);
-    const unsigned da = __hip_atomic_load(&depth[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (da != 0xFFFF) {
-      const unsigned nd = d + da;
-      if (nd > FS_MAX_LEVELS) {
-        atomicOr(&hdr[2], kErrDepth);
-        continue;
-      }
-      __hip_atomic_store(&depth[c], static_cast<uint16_t>(nd), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      atomicAdd(&counts[nd], 1);
-      atomicMax(&hdr[0], static_cast<int>(nd));
-    } else {
-      const unsigned long long ja =
-          __hip_atomic_load(&jump[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const unsigned nd = d + static_cast<unsigned>(ja >> 32);
-      if (nd > FS_MAX_LEVELS + 1u) {
-        atomicOr(&hdr[2], kErrDepth);
-        continue;
-      }
-      __hip_atomic_store(&jump[c],
-                         (static_cast<unsigned long long>(nd) << 32) | (ja & 0xffffffffu),
-                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-}
-
-// Level 0 of GenerateOrder (RB:223-226): every NO_FLOW cell, ring 0
-// included, in ascending index.  One tile = 4096 cells, 16 per thread read
-// as one 16-byte load.
-__global__ __launch_bounds__(kScanBlock) void k_roots(
-    const uint8_t *__restrict__ rcv, int32_t *__restrict__ stack, uint64_t *status,
-    unsigned long long *ticket, unsigned long long tbase, uint32_t epoch, int32_t N,
-    int32_t *hdr) {
-  __shared__ uint32_t sh[kScanBlock / kWave + 1];
-  __shared__ uint32_t s_tile, s_pre;
-  if (threadIdx.x == 0) s_tile = static_cast<uint32_t>(atomicAdd(ticket, 1ull) - tbase);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const long long base = static_cast<long long>(tile) * kRootTile +
-                         static_cast<long long>(threadIdx.x) * kRootItems;
-  uint32_t flags = 0;
-  if (base + kRootItems <= N) {
-    const uint4 v = *reinterpret_cast<const uint4 *>(rcv + base);
-    const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-#pragma unroll
-      for (int b = 0; b < 4; b++)
-        if (((w4[k] >> (8 * b)) & 0xFFu) == kNoFlow) flags |= 1u << (4 * k + b);
-  } else {
-    for (int i = 0; i < kRootItems; i++)
-      if (base + i < N && rcv[base + i] == kNoFlow) flags |= 1u << i;
-  }
-  uint32_t total;
-  const uint32_t ex = block_excl_scan<kScanBlock>(__popc(flags), &total, sh);
-  if (threadIdx.x < kWave) {
-    const uint32_t pre = lookback(status, tile, epoch, total, &hdr[2]);
-    if (threadIdx.x == 0) s_pre = pre;
-  }
-  __syncthreads();
-  long long pos = static_cast<long long>(s_pre) + ex;
-  while (flags) {
-    const int i = __builtin_ctz(flags);
-    flags &= flags - 1;
-    if (pos < N) stack[pos] = static_cast<int32_t>(base + i);
-    pos++;
-  }
-}
-
-// One BFS level of GenerateOrder (RB:234-247): stack[ls, le) -> the donor
-// lists of those cells, in order, written from stack[le].
-__global__ __launch_bounds__(kScanBlock) void k_bfs(
-    int32_t *__restrict__ stack, const uint8_t *__restrict__ dmask, int32_t ls, int32_t le,
-    int W, int32_t N, uint64_t *status, unsigned long long *ticket,
-    unsigned long long tbase, uint32_t epoch, int32_t *hdr) {
-  __shared__ uint32_t sh[kScanBlock / kWave + 1];
-  __shared__ uint32_t s_tile, s_pre;
-  if (threadIdx.x == 0) s_tile = static_cast<uint32_t>(atomicAdd(ticket, 1ull) - tbase);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const long long p0 = static_cast<long long>(ls) + static_cast<long long>(tile) * kBfsTile +
-                       static_cast<long long>(threadIdx.x) * kBfsItems;
-  int32_t cell[kBfsItems];
-  unsigned long long masks = 0;
-  uint32_t cnt = 0;
-#pragma unroll
-  for (int i = 0; i < kBfsItems; i++) {
-    const long long p = p0 + i;
-    int32_t c = 0;
-    uint32_t m = 0;
-    if (p < le) {
-      c = stack[p];
-      if (static_cast<uint32_t>(c) < static_cast<uint32_t>(N)) m = dmask[c];
-      else atomicOr(&hdr[2], kErrIndex);
-    }
-    cell[i] = c;
-    masks |= static_cast<unsigned long long>(m) << (8 * i);
-    cnt += __popc(m);
-  }
-  uint32_t total;
-  const uint32_t ex = block_excl_scan<kScanBlock>(cnt, &total, sh);
-  if (threadIdx.x < kWave) {
-    const uint32_t pre = lookback(status, tile, epoch, total, &hdr[2]);
-    if (threadIdx.x == 0) s_pre = pre;
-  }
-  __syncthreads();
-  long long pos = static_cast<long long>(le) + s_pre + ex;
-#pragma unroll
-  for (int i = 0; i < kBfsItems; i++) {
-    uint32_t m = static_cast<uint32_t>(masks >> (8 * i)) & 0xFFu;
-    while (m) {
-      const int j = __builtin_ctz(m);
-      m &= m - 1;
-      if (pos < N) stack[pos] = cell[i] + slot_offset(j, W);
-      pos++;
-    }
-  }
-}
-
-// Levels [L0, L1) of the BFS in one workgroup, for runs of small levels.
-__global__ __launch_bounds__(kSmallBlock) void k_bfs_small(
-    int32_t *__restrict__ stack, const uint8_t *__restrict__ dmask,
-    const int32_t *__restrict__ levels, int L0, int L1, int W, int32_t N, int32_t *hdr) {
-  __shared__ uint32_t sh[kSmallBlock / kWave + 1];
-  for (int L = L0; L < L1; L++) {
-    const int32_t ls = levels[L], le = levels[L + 1];
-    long long out = le;
-    for (long long b = ls; b < le; b += kSmallLevel) {
-      const long long p0 = b + static_cast<long long>(threadIdx.x) * kSmallItems;
-      int32_t cell[kSmallItems];
-      uint32_t masks = 0, cnt = 0;
-#pragma unroll
-      for (int i = 0; i < kSmallItems; i++) {
-        const long long p = p0 + i;
-        int32_t c = 0;
-        uint32_t m = 0;
-        if (p < le) {
-          c = stack[p];
-          if (static_cast<uint32_t>(c) < static_cast<uint32_t>(N)) m = dmask[c];
-          else atomicOr(&hdr[2], kErrIndex);
-        }
-        cell[i] = c;
-        masks |= m << (8 * i);
-        cnt += __popc(m);
-      }
-      uint32_t total;
-      const uint32_t ex = block_excl_scan<kSmallBlock>(cnt, &total, sh);
-      long long pos = out + ex;
-#pragma unroll
-      for (int i = 0; i < kSmallItems; i++) {
-        uint32_t m = (masks >> (8 * i)) & 0xFFu;
-        while (m) {
-          const int j = __builtin_ctz(m);
-          m &= m - 1;
-          if (pos < N) stack[pos] = cell[i] + slot_offset(j, W);
-          pos++;
-        }
-      }
-      out += total;
-    }
-    __syncthreads();
-  }
-}
-
-// Upstream cell count of one cell: 1 + the counts of its donors, which sit
-// one level higher and are already final.  accum = count * cell_area exactly
-// (RB:262-279: every partial sum there is an integer multiple of cell_area).
-__device__ __forceinline__ void flowacc_cell(int32_t c, const uint8_t *__restrict__ dmask,
-                                             uint32_t *cnt, int W, int32_t N, int32_t *hdr) {
-  if (static_cast<uint32_t>(c) >= static_cast<uint32_t>(N)) {
-    atomicOr(&hdr[2], kErrIndex);
-    return;
-  }
-  uint32_t m = dmask[c];
-  uint32_t s = 1;
-  while (m) {
-    const int j = __builtin_ctz(m);
-    m &= m - 1;
-    s += cnt[c + slot_offset(j, W)];
-  }
-  cnt[c] = s;
-}
-
-__global__ __launch_bounds__(256) void k_flowacc(const int32_t *__restrict__ stack,
-                                                 const uint8_t *__restrict__ dmask,
-                                                 uint32_t *cnt, int32_t ls, int32_t le, int W,
-                                                 int32_t N, int32_t *hdr) {
-  const long long p = static_cast<long long>(ls) + blockIdx.x * 256ll + threadIdx.x;
-  if (p >= le) return;
-  flowacc_cell(stack[p], dmask, cnt, W, N, hdr);
-}
-
-// Levels L_hi down to L_lo (inclusive) in one workgroup.
-__global__ __launch_bounds__(kSmallBlock) void k_flowacc_small(
-    const int32_t *__restrict__ stack, const uint8_t *__restrict__ dmask, uint32_t *cnt,
-    const int32_t *__restrict__ levels, int L_hi, int L_lo, int W, int32_t N, int32_t *hdr) {
-  for (int L = L_hi; L >= L_lo; L--) {
-    const int32_t ls = levels[L], le = levels[L + 1];
-    for (long long p = ls + threadIdx.x; p < le; p += kSmallBlock)
-      flowacc_cell(stack[p], dmask, cnt, W, N, hdr);
-    __syncthreads();
-  }
-}
-
-// AddUplift (RB:285-296) for the interior outlets (level-0 cells away from
-// rings 0 and 1).  Interior cells with a receiver are uplifted in k_erode.
-__global__ __launch_bounds__(256) void k_uplift_roots(const int32_t *__restrict__ stack,
-                                                      int32_t n0, double *h, int W, int H,
-                                                      double ud, int32_t N, int32_t *hdr) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= n0) return;
-  const int32_t c = stack[p];
-  if (static_cast<uint32_t>(c) >= static_cast<uint32_t>(N)) {
-    atomicOr(&hdr[2], kErrIndex);
-    return;
-  }
-  const int y = c / W, x = c - y * W;
-  if (x >= 2 && x < W - 2 && y >= 2 && y < H - 2) h[c] += ud;
-}
-
-// Plain 16-byte-per-thread copy, used only to calibrate hardware traffic
-// counters against a kernel whose HBM bytes are known exactly.
-__global__ __launch_bounds__(256) void k_copy_calibration(const uint4 *__restrict__ src,
-                                                          uint4 *__restrict__ dst, size_t n) {
-  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += gridDim.x * 256ull)
-    dst[i] = src[i];
-}
-
-struct ErodeConst {
-  double kdt;       // keq*dt
-  double lenpow0;   // pow(dx, neq)        cardinal receiver
-  double lenpow1;   // pow(dx*SQRT2, neq)  diagonal receiver
-  double ud;        // ueq*dt
-  double neq;
-  double tol;
-};
-
-// The implicit update of one cell (RB:313-326), operation for operation.
-// NMODE 1: n = 1, where libm returns pow(x,1) = x and pow(x,0) = 1 exactly;
-// NMODE 2: n = 2 with x*x for pow(x,2); NMODE 0: any n through the device
-// pow.  fact = (K*dt)*A^m / dr^n, h0 is the uplifted elevation h + U*dt
-// (RB:294) and hn the receiver's final elevation.
-template <int NMODE>
-__device__ __forceinline__ double newton_update(double h0, double hn, double fact,
-                                                const ErodeConst &k, int32_t *hdr) {
-  double hnew = h0, hp = h0, diff = 2.0 * k.tol;
-  int it = 0;
-  while (fabs(diff) > k.tol) {
-    const double x = hnew - hn;
-    double p1, p0;
-    if (NMODE == 1) {
-      p1 = x;
-      p0 = 1.0;
-    } else if (NMODE == 2) {
-      p1 = x * x;
-      p0 = x;
-    } else {
-      p1 = pow(x, k.neq);
-      p0 = pow(x, k.neq - 1.0);
-    }
-    hnew = hnew - ((hnew - h0) + fact * p1) / (1.0 + (fact * k.neq) * p0);
-    diff = hnew - hp;
-    hp = hnew;
-    if (++it >= kNewtonCap) {
-      atomicOr(&hdr[2], kErrNewton);
-      break;
-    }
-  }
-  return hnew;
-}
-
-// Erode for one stack entry (RB:306-328): h[c] from h[c] + U*dt and the
-// receiver's final elevation (its level was processed earlier).
-template <int NMODE>
-__device__ __forceinline__ void erode_cell(int32_t c, const uint8_t *__restrict__ rcv,
-                                           const uint32_t *__restrict__ cnt,
-                                           const double *__restrict__ ftab, double *h,
-                                           int W, int32_t N, const ErodeConst &k,
-                                           int32_t *hdr) {
-  if (static_cast<uint32_t>(c) >= static_cast<uint32_t>(N)) {
-    atomicOr(&hdr[2], kErrIndex);
-    return;
-  }
-  const int r = rcv[c];
-  if (r >= 8) return;
-  const int32_t n = c + dir_offset(r, W);
-  uint32_t a = cnt[c];
-  if (a > static_cast<uint32_t>(N)) {
-    atomicOr(&hdr[2], kErrIndex);
-    a = static_cast<uint32_t>(N);
-  }
-  h[c] = newton_update<NMODE>(h[c] + k.ud, h[n], ftab[2 * a + (r & 1)], k, hdr);
-}
-
-template <int NMODE>
-__global__ __launch_bounds__(256) void k_erode(const int32_t *__restrict__ stack,
-                                               const uint8_t *__restrict__ rcv,
-                                               const uint32_t *__restrict__ cnt,
-                                               const double *__restrict__ ftab, double *h,
-                                               int32_t ls, int32_t le, int W, int32_t N,
-                                               ErodeConst k, int32_t *hdr) {
-  const long long p = static_cast<long long>(ls) + blockIdx.x * 256ll + threadIdx.x;
-  if (p >= le) return;
-  erode_cell<NMODE>(stack[p], rcv, cnt, ftab, h, W, N, k, hdr);
-}
-
-template <int NMODE>
-__global__ __launch_bounds__(kSmallBlock) void k_erode_small(
-    const int32_t *__restrict__ stack, const uint8_t *__restrict__ rcv,
-    const uint32_t *__restrict__ cnt, const double *__restrict__ ftab, double *h,
-    const int32_t *__restrict__ levels, int L_lo, int L_hi, int W, int32_t N, ErodeConst k,
-    int32_t *hdr) {
-  for (int L = L_lo; L <= L_hi; L++) {
-    const int32_t ls = levels[L], le = levels[L + 1];
-    for (long long p = ls + threadIdx.x; p < le; p += kSmallBlock)
-      erode_cell<NMODE>(stack[p], rcv, cnt, ftab, h, W, N, k, hdr);
-    __syncthreads();
-  }
-}
-
-// ---------------------------------------------------------------------------
-// tile path
-// ---------------------------------------------------------------------------
-// A cell's receiver chain and its whole upstream tree lie within D cells
-// (Chebyshev distance) of it, D being the deepest level of the grid.  So
-// when D <= kHalo, a kTile x kTile tile plus a kHalo-wide halo holds
-// everything the tile's cells depend on, and accumulation and erosion can
-// run level by level inside LDS with one HBM read of the region and one
-// write of the tile.  Every grid cell is a tile cell of exactly one tile.
-constexpr int kTile = 64;
-constexpr int kHalo = 16;
-constexpr int kReg = kTile + 2 * kHalo;       // region side
-constexpr int kRegCells = kReg * kReg;
-constexpr int kAccBlock = 512;
-constexpr int kEroBlock = 1024;
-constexpr uint8_t kOutside = 0xFD;            // region cell beyond the grid
-constexpr uint8_t kEscaped = 0xFF;            // chain leaves the region
-constexpr int kFactCache = 256;               // erosion factors cached in LDS
-
-// Region-local neighbour step in direction r; false if it leaves the region.
-__device__ __forceinline__ bool region_step(int r, int &lx, int &ly) {
-  constexpr unsigned kDx = (0u << 0) | (0u << 2) | (1u << 4) | (2u << 6) |
-                           (2u << 8) | (2u << 10) | (1u << 12) | (0u << 14);
-  constexpr unsigned kDy = (1u << 0) | (0u << 2) | (0u << 4) | (0u << 6) |
-                           (1u << 8) | (2u << 10) | (2u << 12) | (2u << 14);
-  lx += static_cast<int>((kDx >> (2 * r)) & 3u) - 1;
-  ly += static_cast<int>((kDy >> (2 * r)) & 3u) - 1;
-  return lx >= 0 && lx < kReg && ly >= 0 && ly < kReg;
-}
-
-// Workgroups are dealt to the 8 XCDs round-robin by block index; hand each
-// XCD one contiguous run of tiles so that tiles sharing halos share an L2.
-__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
-  const int q = ntiles / 8, rem = ntiles % 8;
-  const int x = b % 8, k = b / 8;
-  return x * q + min(x, rem) + k;
-}
-
-// Load the receivers (and optionally elevations and levels) of the region:
-// every load of a thread is issued before the first LDS store.
-template <int BLOCK, bool WITH_H>
-__device__ __forceinline__ void load_region(const uint8_t *__restrict__ rcv,
-                                            const double *__restrict__ h,
-                                            const uint8_t *__restrict__ lvl,
-                                            const uint32_t *__restrict__ cnt, uint8_t *r_s,
-                                            double *hs, uint8_t *d_s, uint32_t *c_s, int rx0,
-                                            int ry0, int W, int H) {
-  constexpr int kPer = kRegCells / BLOCK;
-  static_assert(kRegCells % BLOCK == 0, "region must split evenly over the block");
-  uint8_t rv[kPer], dv[WITH_H ? kPer : 1];
-  double hv[WITH_H ? kPer : 1];
-  uint32_t cv[WITH_H ? kPer : 1];
-#pragma unroll
-  for (int k = 0; k < kPer; k++) {
-    const int i = k * BLOCK + threadIdx.x;
-    const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
-    const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
-    const size_t g = in ? static_cast<size_t>(gy) * W + gx : 0;
-    rv[k] = in ? rcv[g] : kOutside;
-    if (WITH_H) {
-      hv[k] = in ? h[g] : 0.0;
-      dv[k] = in ? lvl[g] : kEscaped;
-      cv[k] = in ? cnt[g] : 0u;
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < kPer; k++) {
-    const int i = k * BLOCK + threadIdx.x;
-    r_s[i] = rv[k];
-    if (WITH_H) {
-      hs[i] = hv[k];
-      d_s[i] = dv[k];
-      c_s[i] = cv[k];
-    }
-  }
-}
-
-// Pass 1: upstream cell counts (ComputeFlowAcc, RB:262-279) and levels
-// (receiver hops to an outlet, the level GenerateOrder assigns, RB:213-254)
-// of the tile's cells.  Every region cell walks its receiver chain once and
-// adds 1 to each cell it passes, itself included, so a cell's count is the
-// number of region cells draining through it -- its whole upstream area when
-// every level is at most kHalo.  A tile cell deeper than that (or whose chain
-// leaves the region) raises hdr[3]; the host then runs the level path.
-__global__ __launch_bounds__(kAccBlock) void k_tile_acc(const uint8_t *__restrict__ rcv,
-                                                        uint32_t *__restrict__ cnt,
-                                                        uint8_t *__restrict__ lvl, int32_t *hdr,
-                                                        int W, int H, int tiles_x, int ntiles) {
-  __shared__ uint8_t r_s[kRegCells];
-  __shared__ uint32_t c_s[kRegCells];
-  __shared__ int s_over;
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
-  const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
-  if (threadIdx.x == 0) s_over = 0;
-  for (int it = 0; it < kRegCells / kAccBlock; it++) c_s[it * kAccBlock + threadIdx.x] = 0;
-  load_region<kAccBlock, false>(rcv, nullptr, nullptr, nullptr, r_s, nullptr, nullptr, nullptr,
-                                rx0, ry0, W, H);
-  __syncthreads();
-  // kWalkers chains per thread advance together so that their dependent LDS
-  // reads overlap; a walker stops at an outlet, at the region edge, or
-  // after kHalo hops.
-  constexpr int kPer = kRegCells / kAccBlock;
-  constexpr int kWalkers = 6;
-  static_assert(kPer % kWalkers == 0, "walkers must split the thread's cells evenly");
-  int over = 0;
-  for (int g = 0; g < kPer; g += kWalkers) {
-    int wx[kWalkers], wy[kWalkers], hops[kWalkers];
-    uint32_t live = 0, rooted = 0;
-#pragma unroll
-    for (int j = 0; j < kWalkers; j++) {
-      const int i = (g + j) * kAccBlock + threadIdx.x;
-      wx[j] = i % kReg;
-      wy[j] = i / kReg;
-      hops[j] = 0;
-      if (r_s[i] != kOutside) live |= 1u << j;
-    }
-    while (live) {
-#pragma unroll
-      for (int j = 0; j < kWalkers; j++) {
-        if (!(live & (1u << j))) continue;
-        const int x = wy[j] * kReg + wx[j];
-        atomicAdd(&c_s[x], 1u);
-        const int r = r_s[x];
-        if (r >= 8) {
-          rooted |= 1u << j;
-          live &= ~(1u << j);
-        } else if (!region_step(r, wx[j], wy[j]) || ++hops[j] > kHalo) {
-          live &= ~(1u << j);
-        }
-      }
-    }
-    // the level of a tile cell is the length of its walk
-#pragma unroll
-    for (int j = 0; j < kWalkers; j++) {
-      const int i = (g + j) * kAccBlock + threadIdx.x;
-      const int tx = i % kReg - kHalo, ty = i / kReg - kHalo;
-      if (r_s[i] == kOutside || tx < 0 || tx >= kTile || ty < 0 || ty >= kTile) continue;
-      const size_t gi = static_cast<size_t>(y0 + ty) * W + (x0 + tx);
-      if (rooted & (1u << j)) lvl[gi] = static_cast<uint8_t>(hops[j]);
-      else over = 1;
-    }
-  }
-  if (over) s_over = 1;
-  __syncthreads();
-  for (int it = 0; it < kTile * kTile / kAccBlock; it++) {
-    const int i = it * kAccBlock + threadIdx.x;
-    const int tx = i % kTile, ty = i / kTile;
-    const int gx = x0 + tx, gy = y0 + ty;
-    if (gx >= W || gy >= H) continue;
-    cnt[static_cast<size_t>(gy) * W + gx] = c_s[(ty + kHalo) * kReg + tx + kHalo];
-  }
-  if (threadIdx.x == 0 && s_over) atomicAdd(&hdr[3], 1);
-}
-
-// Pass 2: uplift and erosion (AddUplift RB:285-296, Erode RB:306-328) of the
-// tile's cells.  Levels come from pass 1.  Region cells are bucketed by
-// level into a list whose level segments are padded to whole 64-entry
-// chunks; each wave takes chunks in increasing order and each lane solves
-// its cell with the reference's Newton iteration on LDS-resident elevations
-// as soon as its receiver (one level lower, hence in an earlier chunk) is
-// marked done.  With one level per chunk no lane waits on a lane of its own
-// wave, and the lowest chunk in progress never waits on an unfinished one,
-// so the wavefront always advances; the wait is bounded all the same and
-// reported in hdr[2].  The tile cells' receiver chains lie in the region
-// because every level is at most kHalo.  Reads h_old, writes h_new:
-// neighbouring tiles read this tile's cells as their halo.
-constexpr uint8_t kDone = 0xFD;
-constexpr int kListCap = kRegCells + (kHalo + 1) * (kWave - 1);
-template <int NMODE>
-__global__ __launch_bounds__(kEroBlock) void k_tile_erode(
-    const double *__restrict__ h_old, double *__restrict__ h_new,
-    const uint8_t *__restrict__ rcv, const uint8_t *__restrict__ lvl,
-    const uint32_t *__restrict__ cnt, const double *__restrict__ ftab, int W, int H,
-    int tiles_x, int ntiles, ErodeConst k, int32_t *hdr) {
-  constexpr int kWaves = kEroBlock / kWave;
-  constexpr int kKeys = kHalo + 1;
-  constexpr int kPer = kRegCells / kEroBlock;
-  __shared__ double hs[kRegCells];
-  __shared__ uint8_t r_s[kRegCells];
-  __shared__ uint8_t d_s[kRegCells];       // level, then kDone once solved
-  __shared__ uint16_t list[kListCap];
-  __shared__ uint32_t c_s[kRegCells];
-  __shared__ double ft_s[2 * kFactCache];
-  __shared__ int wcount[kKeys][kWaves];
-  __shared__ int ltotal[kKeys], loff[kKeys + 1];
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
-  const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
-  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  for (int i = threadIdx.x; i < kKeys * kWaves; i += kEroBlock) (&wcount[0][0])[i] = 0;
-  if (threadIdx.x < 2 * kFactCache) ft_s[threadIdx.x] = ftab[threadIdx.x];
-  load_region<kEroBlock, true>(rcv, h_old, lvl, cnt, r_s, hs, d_s, c_s, rx0, ry0, W, H);
-  __syncthreads();
-  // bucket the region cells by level: rank within (level, wave), then bases
-  int rank[kPer];
-#pragma unroll
-  for (int q = 0; q < kPer; q++) {
-    const int d = d_s[q * kEroBlock + threadIdx.x];
-    rank[q] = d < kKeys ? atomicAdd(&wcount[d][wave], 1) : -1;
-  }
-  __syncthreads();
-  if (threadIdx.x < kKeys) {
-    int run = 0;
-    for (int w = 0; w < kWaves; w++) {
-      const int c = wcount[threadIdx.x][w];
-      wcount[threadIdx.x][w] = run;
-      run += c;
-    }
-    ltotal[threadIdx.x] = run;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int L = 0; L < kKeys; L++) {
-      loff[L] = run;
-      run += (ltotal[L] + kWave - 1) / kWave * kWave;   // whole chunks per level
-    }
-    loff[kKeys] = run;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < kPer; q++) {
-    const int i = q * kEroBlock + threadIdx.x;
-    const int d = d_s[i];
-    if (rank[q] >= 0) list[loff[d] + wcount[d][wave] + rank[q]] = static_cast<uint16_t>(i);
-  }
-  if (threadIdx.x < kKeys)
-    for (int e = loff[threadIdx.x] + ltotal[threadIdx.x]; e < loff[threadIdx.x + 1]; e++)
-      list[e] = 0xFFFF;
-  __syncthreads();
-  const int nchunks = loff[kKeys] / kWave, root_chunks = loff[1] / kWave;
-  for (int ch = wave; ch < nchunks; ch += kWaves) {
-    const int i = list[ch * kWave + lane];
-    if (i == 0xFFFF) continue;
-    if (ch < root_chunks) {
-      // level 0: uplift of interior outlets; rings 0 and 1 stay fixed
-      const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
-      if (gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2) hs[i] += k.ud;
-    } else {
-      const int r = r_s[i];
-      int lx = i % kReg, ly = i / kReg;
-      if (region_step(r, lx, ly)) {          // else the receiver is not needed here
-        const int nb = ly * kReg + lx;
-        int spins = 0;
-        while (__hip_atomic_load(&d_s[nb], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) !=
-               kDone) {
-          if (++spins > (1 << 22)) {
-            atomicOr(&hdr[2], kErrLookback);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-        }
-        const uint32_t a = c_s[i];
-        const int ti = 2 * static_cast<int>(a) + (r & 1);
-        const double fact = a < kFactCache ? ft_s[ti] : ftab[ti];
-        hs[i] = newton_update<NMODE>(hs[i] + k.ud, hs[nb], fact, k, hdr);
-      }
-    }
-    __hip_atomic_store(&d_s[i], kDone, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-  }
-  __syncthreads();
-  for (int it = 0; it < kTile * kTile / kEroBlock; it++) {
-    const int i = it * kEroBlock + threadIdx.x;
-    const int tx = i % kTile, ty = i / kTile;
-    const int gx = x0 + tx, gy = y0 + ty;
-    if (gx >= W || gy >= H) continue;
-    h_new[static_cast<size_t>(gy) * W + gx] = hs[(ty + kHalo) * kReg + tx + kHalo];
-  }
-}
-
-}  // namespace
+#include "fs_device.h"
 
 // ===========================================================================
 // host side
 // ===========================================================================
-#define FS_HIP_TRY(expr)                                                           \
-  do {                                                                             \
-    const hipError_t e_ = (expr);                                                  \
-    if (e_ != hipSuccess) {                                                        \
-      fs::set_last_error(std::string(#expr) + ": " + hipGetErrorString(e_));       \
-      return FS_ERR_HIP;                                                           \
-    }                                                                              \
-  } while (0)
-
-#define FS_TRY(expr)              \
-  do {                            \
-    const int s_ = (expr);        \
-    if (s_ != FS_OK) return s_;   \
-  } while (0)
-
 struct KTimed {
   int kid;
   hipEvent_t a, b;
@@ -1079,17 +199,6 @@ void free_model(fs_model *m) {
   delete m;
 }
 
-template <class T>
-int dev_alloc(T **ptr, size_t count) {
-  const hipError_t e = hipMalloc(reinterpret_cast<void **>(ptr), std::max<size_t>(count, 1) * sizeof(T));
-  if (e != hipSuccess) {
-    *ptr = nullptr;
-    fs::set_last_error(std::string("hipMalloc: ") + hipGetErrorString(e));
-    return FS_ERR_OUT_OF_MEMORY;
-  }
-  return FS_OK;
-}
-
 int validate(int W, int H, const fs_params *p) {
   if (!p) return FS_ERR_INVALID_ARGUMENT;
   if (W < 5 || H < 5 || static_cast<long long>(W) * H >= (1ll << 31)) return FS_ERR_DIMENSIONS;
@@ -1140,23 +249,23 @@ int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_
 
   const size_t N = static_cast<size_t>(m->N);
   const size_t ntiles = (N + kBfsTile - 1) / kBfsTile + 1;
-  FS_TRY(dev_alloc(&m->h, N));
-  FS_TRY(dev_alloc(&m->h_alt, N));
-  FS_TRY(dev_alloc(&m->rcv, N + 16));
-  FS_TRY(dev_alloc(&m->dmask, N));
-  FS_TRY(dev_alloc(&m->lvl, N));
-  FS_TRY(dev_alloc(&m->depth, N));
-  FS_TRY(dev_alloc(&m->jump, N));
-  FS_TRY(dev_alloc(&m->list, N));
-  FS_TRY(dev_alloc(&m->stack, N));
-  FS_TRY(dev_alloc(&m->cnt, N));
+  FS_TRY(fs::dev_alloc(&m->h, N));
+  FS_TRY(fs::dev_alloc(&m->h_alt, N));
+  FS_TRY(fs::dev_alloc(&m->rcv, N + 16));
+  FS_TRY(fs::dev_alloc(&m->dmask, N));
+  FS_TRY(fs::dev_alloc(&m->lvl, N));
+  FS_TRY(fs::dev_alloc(&m->depth, N));
+  FS_TRY(fs::dev_alloc(&m->jump, N));
+  FS_TRY(fs::dev_alloc(&m->list, N));
+  FS_TRY(fs::dev_alloc(&m->stack, N));
+  FS_TRY(fs::dev_alloc(&m->cnt, N));
   // at least kFactCache counts, so the tile kernels' LDS cache stays in bounds
   const size_t kmax = std::max<size_t>(N, kFactCache - 1);
-  FS_TRY(dev_alloc(&m->ftab, 2 * (kmax + 1)));
-  FS_TRY(dev_alloc(&m->hdr, kHdrInts + FS_MAX_LEVELS + 3));
-  FS_TRY(dev_alloc(&m->levels_d, FS_MAX_LEVELS + 4));
-  FS_TRY(dev_alloc(&m->status, ntiles));
-  FS_TRY(dev_alloc(&m->ticket, 1));
+  FS_TRY(fs::dev_alloc(&m->ftab, 2 * (kmax + 1)));
+  FS_TRY(fs::dev_alloc(&m->hdr, kHdrInts + FS_MAX_LEVELS + 3));
+  FS_TRY(fs::dev_alloc(&m->levels_d, FS_MAX_LEVELS + 4));
+  FS_TRY(fs::dev_alloc(&m->status, ntiles));
+  FS_TRY(fs::dev_alloc(&m->ticket, 1));
   if (hipHostMalloc(reinterpret_cast<void **>(&m->hdr_h),
                     (kHdrInts + FS_MAX_LEVELS + 3) * sizeof(int32_t)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void **>(&m->levels_h),
@@ -1722,8 +831,8 @@ int fs_calibrate_copy(size_t nbytes, int device, double *ms) {
   FS_HIP_TRY(hipSetDevice(device));
   const size_t n = nbytes / 16;
   uint4 *a = nullptr, *b = nullptr;
-  FS_TRY(dev_alloc(&a, n));
-  if (dev_alloc(&b, n) != FS_OK) {
+  FS_TRY(fs::dev_alloc(&a, n));
+  if (fs::dev_alloc(&b, n) != FS_OK) {
     (void)hipFree(a);
     return FS_ERR_OUT_OF_MEMORY;
   }
