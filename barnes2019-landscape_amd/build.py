@@ -59,7 +59,8 @@ def build(verbose=False):
     os.makedirs(LIB_DIR, exist_ok=True)
     tmp = LIB + ".tmp"
     _run([HIPCC, *COMMON, *DEVICE, "-shared", "-I" + INCLUDE, "-I" + CSRC,
-          os.path.join(CSRC, "fastscape_hip.hip"), os.path.join(CSRC, "fs_host.cpp"),
+          os.path.join(CSRC, "fastscape_hip.hip"), os.path.join(CSRC, "fs_ensemble.hip"),
+          os.path.join(CSRC, "fs_host.cpp"),
           "-o", tmp], verbose)
     os.replace(tmp, LIB)
     cxx = shutil.which("g++") or "g++"

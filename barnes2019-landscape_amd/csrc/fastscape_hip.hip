@@ -199,23 +199,11 @@ void free_model(fs_model *m) {
   delete m;
 }
 
-int validate(int W, int H, const fs_params *p) {
-  if (!p) return FS_ERR_INVALID_ARGUMENT;
-  if (W < 5 || H < 5 || static_cast<long long>(W) * H >= (1ll << 31)) return FS_ERR_DIMENSIONS;
-  if (!(p->dx > 0) || !(p->cell_area > 0) || !(p->tol > 0) || !std::isfinite(p->keq) ||
-      !std::isfinite(p->neq) || !std::isfinite(p->meq) || !std::isfinite(p->ueq) ||
-      !std::isfinite(p->dt))
-    return FS_ERR_INVALID_ARGUMENT;
-  // accum = count*cell_area must stay an exact integer-valued double
-  if (static_cast<double>(W) * H * p->cell_area > 9007199254740992.0) return FS_ERR_INVALID_ARGUMENT;
-  return FS_OK;
-}
-
 int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_t seed,
                  int device, fs_model **out) {
   if (!out) return FS_ERR_INVALID_ARGUMENT;
   *out = nullptr;
-  FS_TRY(validate(W, H, p));
+  FS_TRY(fs::validate_params(W, H, p));
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev) {
     fs::set_last_error("no HIP device " + std::to_string(device));
@@ -235,13 +223,7 @@ int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_
   m->p = *p;
   m->d_card = p->dx;
   m->d_diag = p->dx * fs::kSqrt2;
-  m->ek.kdt = p->keq * p->dt;
-  m->ek.lenpow0 = std::pow(m->d_card, p->neq);
-  m->ek.lenpow1 = std::pow(m->d_diag, p->neq);
-  m->ek.ud = p->ueq * p->dt;
-  m->ek.neq = p->neq;
-  m->ek.tol = p->tol;
-  m->nmode = (p->neq == 1.0) ? 1 : (p->neq == 2.0) ? 2 : 0;
+  m->ek = make_erode_const(*p, &m->nmode);
   FS_HIP_TRY(hipSetDevice(device));
   FS_HIP_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
   for (auto &set : m->ev)

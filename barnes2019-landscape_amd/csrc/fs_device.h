@@ -6,10 +6,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 
 #include "fastscape.h"
+#include "fs_internal.h"
 #include "fs_receiver.h"
 
 namespace {
@@ -528,6 +530,19 @@ struct ErodeConst {
   double tol;
 };
 
+// Per-model constants of the erosion (RB:313-315) and the pow mode of neq.
+inline ErodeConst make_erode_const(const fs_params &p, int *nmode) {
+  ErodeConst k;
+  k.kdt = p.keq * p.dt;
+  k.lenpow0 = std::pow(p.dx, p.neq);
+  k.lenpow1 = std::pow(p.dx * fs::kSqrt2, p.neq);
+  k.ud = p.ueq * p.dt;
+  k.neq = p.neq;
+  k.tol = p.tol;
+  *nmode = (p.neq == 1.0) ? 1 : (p.neq == 2.0) ? 2 : 0;
+  return k;
+}
+
 // The implicit update of one cell (RB:313-326), operation for operation.
 // NMODE 1: n = 1, where libm returns pow(x,1) = x and pow(x,0) = 1 exactly;
 // NMODE 2: n = 2 with x*x for pow(x,2); NMODE 0: any n through the device
@@ -699,27 +714,31 @@ struct AccShared {
   uint8_t r_s[kRegCells];
   uint32_t c_s[kRegCells];
   int over;
+  int hist[kHalo + 1];
 };
 
 // Returns nonzero (block-uniform) when a tile cell is deeper than kHalo.
+// With level_hist, also adds the tile cells' level histogram to it.
+template <int BLOCK>
 __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__restrict__ rcv,
                                              uint32_t *__restrict__ cnt,
                                              uint8_t *__restrict__ lvl, int W, int H, int t,
-                                             int tiles_x) {
+                                             int tiles_x, int32_t *level_hist) {
   uint8_t *r_s = sh.r_s;
   uint32_t *c_s = sh.c_s;
   const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
   const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
   if (threadIdx.x == 0) sh.over = 0;
-  for (int it = 0; it < kRegCells / kAccBlock; it++) c_s[it * kAccBlock + threadIdx.x] = 0;
-  load_region<kAccBlock, false>(rcv, nullptr, nullptr, nullptr, r_s, nullptr, nullptr, nullptr,
-                                rx0, ry0, W, H);
+  if (threadIdx.x < kHalo + 1) sh.hist[threadIdx.x] = 0;
+  for (int it = 0; it < kRegCells / BLOCK; it++) c_s[it * BLOCK + threadIdx.x] = 0;
+  load_region<BLOCK, false>(rcv, nullptr, nullptr, nullptr, r_s, nullptr, nullptr, nullptr,
+                            rx0, ry0, W, H);
   __syncthreads();
   // kWalkers chains per thread advance together so that their dependent LDS
   // reads overlap; a walker stops at an outlet, at the region edge, or
   // after kHalo hops.
-  constexpr int kPer = kRegCells / kAccBlock;
-  constexpr int kWalkers = 6;
+  constexpr int kPer = kRegCells / BLOCK;
+  constexpr int kWalkers = BLOCK == 512 ? 6 : kPer;
   static_assert(kPer % kWalkers == 0, "walkers must split the thread's cells evenly");
   int over = 0;
   for (int g = 0; g < kPer; g += kWalkers) {
@@ -727,7 +746,7 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
     uint32_t live = 0, rooted = 0;
 #pragma unroll
     for (int j = 0; j < kWalkers; j++) {
-      const int i = (g + j) * kAccBlock + threadIdx.x;
+      const int i = (g + j) * BLOCK + threadIdx.x;
       wx[j] = i % kReg;
       wy[j] = i / kReg;
       hops[j] = 0;
@@ -751,18 +770,24 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
     // the level of a tile cell is the length of its walk
 #pragma unroll
     for (int j = 0; j < kWalkers; j++) {
-      const int i = (g + j) * kAccBlock + threadIdx.x;
+      const int i = (g + j) * BLOCK + threadIdx.x;
       const int tx = i % kReg - kHalo, ty = i / kReg - kHalo;
       if (r_s[i] == kOutside || tx < 0 || tx >= kTile || ty < 0 || ty >= kTile) continue;
       const size_t gi = static_cast<size_t>(y0 + ty) * W + (x0 + tx);
-      if (rooted & (1u << j)) lvl[gi] = static_cast<uint8_t>(hops[j]);
-      else over = 1;
+      if (rooted & (1u << j)) {
+        lvl[gi] = static_cast<uint8_t>(hops[j]);
+        if (level_hist) atomicAdd(&sh.hist[hops[j]], 1);
+      } else {
+        over = 1;
+      }
     }
   }
   if (over) sh.over = 1;
   __syncthreads();
-  for (int it = 0; it < kTile * kTile / kAccBlock; it++) {
-    const int i = it * kAccBlock + threadIdx.x;
+  if (level_hist && threadIdx.x < kHalo + 1 && sh.hist[threadIdx.x])
+    atomicAdd(&level_hist[threadIdx.x], sh.hist[threadIdx.x]);
+  for (int it = 0; it < kTile * kTile / BLOCK; it++) {
+    const int i = it * BLOCK + threadIdx.x;
     const int tx = i % kTile, ty = i / kTile;
     const int gx = x0 + tx, gy = y0 + ty;
     if (gx >= W || gy >= H) continue;
@@ -776,7 +801,8 @@ __global__ __launch_bounds__(kAccBlock) void k_tile_acc(const uint8_t *__restric
                                                         uint8_t *__restrict__ lvl, int32_t *hdr,
                                                         int W, int H, int tiles_x, int ntiles) {
   __shared__ AccShared sh;
-  const int over = tile_acc_body(sh, rcv, cnt, lvl, W, H, xcd_tile(blockIdx.x, ntiles), tiles_x);
+  const int over = tile_acc_body<kAccBlock>(sh, rcv, cnt, lvl, W, H,
+                                            xcd_tile(blockIdx.x, ntiles), tiles_x, nullptr);
   if (threadIdx.x == 0 && over) atomicAdd(&hdr[3], 1);
 }
 

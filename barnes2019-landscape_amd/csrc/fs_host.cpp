@@ -21,6 +21,18 @@ thread_local std::string g_last_error;
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 const char *last_error() { return g_last_error.c_str(); }
 
+int validate_params(int W, int H, const fs_params *p) {
+  if (!p) return FS_ERR_INVALID_ARGUMENT;
+  if (W < 5 || H < 5 || static_cast<long long>(W) * H >= (1ll << 31)) return FS_ERR_DIMENSIONS;
+  if (!(p->dx > 0) || !(p->cell_area > 0) || !(p->tol > 0) || !std::isfinite(p->keq) ||
+      !std::isfinite(p->neq) || !std::isfinite(p->meq) || !std::isfinite(p->ueq) ||
+      !std::isfinite(p->dt))
+    return FS_ERR_INVALID_ARGUMENT;
+  // accum = count*cell_area must stay an exact integer-valued double
+  if (static_cast<double>(W) * H * p->cell_area > 9007199254740992.0) return FS_ERR_INVALID_ARGUMENT;
+  return FS_OK;
+}
+
 // mt19937 (Matsumoto & Nishimura 1998), the engine behind random.hpp:22.
 struct Mt19937 {
   uint32_t mt[624];

@@ -13,6 +13,8 @@ namespace fs {
 
 void set_last_error(const std::string &msg);
 const char *last_error();
+// Grid dimensions and parameter block accepted by the model (status code).
+int validate_params(int W, int H, const fs_params *p);
 void generate_terrain(double *h, int W, int H, uint32_t engine_seed);
 void fill_fact_table(double *tab, size_t kmax, double cell_area, double meq, double kdt,
                      double lenpow_card, double lenpow_diag);

@@ -6,9 +6,12 @@ benchmarks and scripting.
 """
 from ._lib import (EXPORTED, FS_MAX_LEVELS, FS_NO_FLOW, KERNELS, LIB_PATH, PATH_AUTO,
                    PATH_LEVELS, PRESET_BENCHMARK, PRESET_REFERENCE, TILE_HALO, TIMER_NAMES, FastScape, FastScapeError, FsParams,
-                   calibrate_copy, generate_terrain, library, preset, write_dem)
+                   FsRealization, FsSchedule, calibrate_copy, generate_terrain, library, preset,
+                   write_dem)
+from .ensemble import Ensemble, config4_realizations, realization, schedule
 
 __all__ = ["EXPORTED", "FS_MAX_LEVELS", "FS_NO_FLOW", "KERNELS", "LIB_PATH", "PATH_AUTO",
            "PATH_LEVELS", "TILE_HALO",
            "PRESET_BENCHMARK", "PRESET_REFERENCE", "TIMER_NAMES", "FastScape",
-           "FastScapeError", "FsParams", "calibrate_copy", "generate_terrain", "library", "preset", "write_dem"]
+           "FastScapeError", "FsParams", "FsRealization", "FsSchedule", "Ensemble",
+           "config4_realizations", "realization", "schedule", "calibrate_copy", "generate_terrain", "library", "preset", "write_dem"]

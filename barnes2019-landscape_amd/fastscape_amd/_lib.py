@@ -33,7 +33,13 @@ EXPORTED = [
     "fs_get_h", "fs_set_h", "fs_get_timers", "fs_reset_timers", "fs_get_receivers",
     "fs_get_donors", "fs_get_stack", "fs_get_levels", "fs_get_accum", "fs_device_h",
     "fs_stream", "fs_synchronize", "fs_set_step_path", "fs_path_stats", "fs_set_kernel_timing", "fs_kernel_stats",
-    "fs_kernel_name", "fs_calibrate_copy", "fs_write_dem", "fs_generate_terrain", "fs_strerror", "fs_last_error",
+    "fs_kernel_name", "fs_calibrate_copy", "fs_ensemble_create", "fs_ensemble_destroy",
+    "fs_ensemble_size", "fs_ensemble_run", "fs_ensemble_step", "fs_ensemble_get_h",
+    "fs_ensemble_set_h", "fs_ensemble_get_receivers", "fs_ensemble_get_accum",
+    "fs_ensemble_get_levels", "fs_ensemble_path_stats", "fs_ensemble_set_schedule",
+    "fs_ensemble_get_schedule", "fs_ensemble_autotune", "fs_ensemble_tune_log",
+    "fs_ensemble_stats", "fs_ensemble_synchronize",
+    "fs_write_dem", "fs_generate_terrain", "fs_strerror", "fs_last_error",
 ]
 
 
@@ -42,6 +48,21 @@ class FsParams(ctypes.Structure):
     _fields_ = [("keq", ctypes.c_double), ("neq", ctypes.c_double), ("meq", ctypes.c_double),
                 ("ueq", ctypes.c_double), ("dt", ctypes.c_double), ("dx", ctypes.c_double),
                 ("cell_area", ctypes.c_double), ("tol", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FsRealization(ctypes.Structure):
+    """fs_realization: grid size, parameter block, terrain seed."""
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("params", FsParams),
+                ("engine_seed", ctypes.c_uint32)]
+
+
+class FsSchedule(ctypes.Structure):
+    """fs_schedule: receiver rows per item, tiles per claim, cells per level-path workgroup."""
+    _fields_ = [("recv_rows", ctypes.c_int), ("tiles_per_claim", ctypes.c_int),
+                ("group_cells", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -97,6 +118,25 @@ def _lib():
         "fs_kernel_stats": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _u64p]),
         "fs_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
         "fs_calibrate_copy": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, _dp]),
+        "fs_ensemble_create": (ctypes.c_int, [ctypes.POINTER(FsRealization), ctypes.c_int,
+                                              ctypes.c_int, ctypes.POINTER(_vp)]),
+        "fs_ensemble_destroy": (ctypes.c_int, [_vp]),
+        "fs_ensemble_size": (ctypes.c_int, [_vp, _ip, ctypes.POINTER(ctypes.c_size_t)]),
+        "fs_ensemble_run": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "fs_ensemble_step": (ctypes.c_int, [_vp]),
+        "fs_ensemble_get_h": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_size_t]),
+        "fs_ensemble_set_h": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_size_t]),
+        "fs_ensemble_get_receivers": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.c_size_t]),
+        "fs_ensemble_get_accum": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_size_t]),
+        "fs_ensemble_get_levels": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.c_size_t, _ip]),
+        "fs_ensemble_path_stats": (ctypes.c_int, [_vp, ctypes.c_int, _u64p, _u64p]),
+        "fs_ensemble_set_schedule": (ctypes.c_int, [_vp, ctypes.POINTER(FsSchedule)]),
+        "fs_ensemble_get_schedule": (ctypes.c_int, [_vp, ctypes.POINTER(FsSchedule)]),
+        "fs_ensemble_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+        "fs_ensemble_tune_log": (ctypes.c_int, [_vp, ctypes.POINTER(FsSchedule), _dp,
+                                                ctypes.c_int, _ip]),
+        "fs_ensemble_stats": (ctypes.c_int, [_vp, _u64p, _dp]),
+        "fs_ensemble_synchronize": (ctypes.c_int, [_vp]),
         "fs_write_dem": (ctypes.c_int, [ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int]),
         "fs_generate_terrain": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32]),
         "fs_strerror": (ctypes.c_char_p, [ctypes.c_int]),

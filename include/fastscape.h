@@ -197,6 +197,64 @@ const char *fs_kernel_name(int kernel);
  * launch reads and writes exactly nbytes). */
 int fs_calibrate_copy(size_t nbytes, int device, double *ms);
 
+/* ---- ensembles --------------------------------------------------------------
+ * Many independent grids ("realizations") of different sizes and parameters,
+ * advanced together by one persistent cooperative launch per time-step.  No
+ * reference counterpart: the reference runs one grid per process; its README
+ * motivates ensembles for inverse problems (README.md:28-29).  Each
+ * realization's time-step is the reference's (RB:366-376) and gives the same
+ * bits as an fs_model of the same size, parameters and initial terrain.
+ * Realization r's initial terrain is GenerateRandomTerrain (RB:111-130) from
+ * an mt19937 seeded with its engine_seed (0 = the reference's terrain). */
+typedef struct fs_ensemble fs_ensemble;
+typedef struct fs_realization {
+  int width, height;
+  fs_params params;
+  uint32_t engine_seed;
+} fs_realization;
+
+/* Work schedule of a step; every schedule gives the same results.
+ *   recv_rows        rows of 256 columns per receiver work item (16..256,
+ *                    a multiple of 16)
+ *   tiles_per_claim  64x64 tiles per erosion work item (1..16)
+ *   group_cells      cells per workgroup of a deep realization's level
+ *                    path: ceil(W*H / group_cells) workgroups (at most 16)
+ *                    share it; 0 = one workgroup */
+typedef struct fs_schedule {
+  int recv_rows;
+  int tiles_per_claim;
+  int group_cells;
+} fs_schedule;
+
+int fs_ensemble_create(const fs_realization *reals, int count, int device, fs_ensemble **out);
+int fs_ensemble_destroy(fs_ensemble *e);
+int fs_ensemble_size(fs_ensemble *e, int *count, size_t *cells);
+/* nstep+1 time-steps of every realization (RB:366), resumable. */
+int fs_ensemble_run(fs_ensemble *e, int nstep);
+int fs_ensemble_step(fs_ensemble *e);
+int fs_ensemble_get_h(fs_ensemble *e, int r, double *h, size_t count);
+int fs_ensemble_set_h(fs_ensemble *e, int r, const double *h, size_t count);
+/* State of realization r after the last step, in fs_get_* conventions. */
+int fs_ensemble_get_receivers(fs_ensemble *e, int r, int32_t *rec, size_t count);
+int fs_ensemble_get_accum(fs_ensemble *e, int r, double *accum, size_t count);
+int fs_ensemble_get_levels(fs_ensemble *e, int r, int32_t *levels, size_t capacity,
+                           int32_t *nlevel);
+int fs_ensemble_path_stats(fs_ensemble *e, int r, uint64_t *tile_steps, uint64_t *level_steps);
+int fs_ensemble_set_schedule(fs_ensemble *e, const fs_schedule *s);
+int fs_ensemble_get_schedule(fs_ensemble *e, fs_schedule *s);
+/* Online autotuning inside fs_ensemble_run: at the run's first step and
+ * then every `interval` steps (0 = only at the first), each candidate
+ * schedule of a coordinate search runs `trial_steps` real time-steps and
+ * the fastest is kept.  trial_steps = 0 turns tuning off (the default). */
+int fs_ensemble_autotune(fs_ensemble *e, int trial_steps, int interval);
+/* Tuning record: candidates timed so far and the ms/step of each; pass
+ * NULL arrays to query the count. */
+int fs_ensemble_tune_log(fs_ensemble *e, fs_schedule *cand, double *ms_per_step,
+                         int capacity, int *count);
+/* Steps taken and summed device milliseconds of their launches. */
+int fs_ensemble_stats(fs_ensemble *e, uint64_t *steps, double *step_ms);
+int fs_ensemble_synchronize(fs_ensemble *e);
+
 /* ---- host helpers ---------------------------------------------------------- */
 /* PrintDEM (RB:20-40): ArcGIS ASCII grid of the inner (W-2)x(H-2) cells,
  * byte-identical to the reference writer. */

@@ -134,3 +134,34 @@ def test_no_oracle_in_product_path():
             if f.endswith((".py", ".hip", ".cpp", ".h", ".hpp")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "liboracle" not in text and "import oracle" not in text, f
+
+
+def test_ensemble_argument_errors_without_device():
+    lib = fa.library()
+    e = ctypes.c_void_p()
+    assert lib.fs_ensemble_create(None, 0, 0, ctypes.byref(e)) == 1
+    bad = (fa.FsRealization * 2)(fa.realization(64, 64, fa.preset(fa.PRESET_BENCHMARK)),
+                                 fa.realization(3, 64, fa.preset(fa.PRESET_BENCHMARK)))
+    assert lib.fs_ensemble_create(bad, 2, 0, ctypes.byref(e)) == 2
+    nan = fa.preset(fa.PRESET_BENCHMARK, keq=float("nan"))
+    one = (fa.FsRealization * 1)(fa.realization(64, 64, nan))
+    assert lib.fs_ensemble_create(one, 1, 0, ctypes.byref(e)) == 1
+    assert lib.fs_ensemble_destroy(None) == 1
+    assert lib.fs_ensemble_run(None, 1) == 1
+
+
+def test_ensemble_workload_sampler():
+    reals, meta = fa.config4_realizations(256)
+    again, meta2 = fa.config4_realizations(256)
+    assert meta == meta2
+    assert [m["seed"] for m in meta] == list(range(256))
+    assert {m["side"] for m in meta} <= {512, 768, 1024, 1536, 2048}
+    K = np.array([m["K"] for m in meta])
+    U = np.array([m["U"] for m in meta])
+    mm = np.array([m["m"] for m in meta])
+    assert K.min() >= 1e-6 and K.max() <= 1e-4
+    assert U.min() >= 5e-4 * (1 - 1e-12) and U.max() <= 5e-3 * (1 + 1e-12)
+    assert mm.min() >= 0.3 and mm.max() <= 0.6
+    for r, m in zip(reals, meta):
+        assert r.width == r.height == m["side"]
+        assert r.params.neq == 1.0 and r.params.keq == m["K"] and r.params.dx == 100.0
