@@ -9,6 +9,8 @@ reference's terrain for any nonzero seed), state resident in HBM before the
 timed region.
 
     python bench.py [--gpus 1] [--steps 121] [--warmup 5] [--size 10000]
+    python bench.py --workload ensemble [--steps 500]     # 256 realizations
+
 
 Prints one JSON line.  `value` is cell-steps/s over the timed steps;
 `roofline` is the dominant kernel's algorithmic bytes / its HIP-event time;
@@ -47,8 +49,16 @@ KERNEL_BYTES = {
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=121)
+    ap.add_argument("--workload", choices=["grid", "ensemble"], default="grid",
+                    help="grid: one 10000^2 grid (default); ensemble: 256 realizations of "
+                         "sides 512..2048 with sampled K, U, m")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (grid: 121; ensemble: 500)")
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--realizations", type=int, default=256)
+    ap.add_argument("--tune-trial", type=int, default=1,
+                    help="ensemble: steps per autotuner candidate (0 = no tuning)")
+    ap.add_argument("--tune-interval", type=int, default=100)
     ap.add_argument("--size", type=int, default=10000)
     ap.add_argument("--profile-steps", type=int, default=10,
                     help="steps timed per kernel with HIP events for the roofline")
@@ -84,6 +94,66 @@ def cpu_baseline(size, nrun):
                 "fastscape_RB+P.cpp" if kind == "reference" else "oracle RB+P restatement")}
 
 
+def bench_ensemble(args, fa):
+    """The ensemble workload: `--realizations` grids with square sides drawn
+    from {512, 768, 1024, 1536, 2048}, K log-uniform [1e-6, 1e-4], U
+    log-uniform [5e-4, 5e-3], m uniform [0.3, 0.6], n = 1, terrain seed =
+    realization index; every step is one persistent launch over all of them.
+    The autotuner runs inside the warmup and every --tune-interval steps
+    inside the timed region (its trial steps are real steps)."""
+    steps = args.steps or 500
+    reals, meta = fa.config4_realizations(args.realizations, master_seed=0)
+    ens = fa.Ensemble(reals)
+    cells = ens.cells
+    ens.autotune(args.tune_trial, args.tune_interval)
+    for _ in range(args.warmup):
+        ens.step()
+    ens.synchronize()
+    s0, ms0 = ens.stats()
+    t0 = time.perf_counter()
+    ens.run(steps - 1)          # run(n) = n+1 time-steps
+    ens.synchronize()
+    t1 = time.perf_counter()
+    s1, ms1 = ens.stats()
+    secs = t1 - t0
+    value = cells * steps / secs
+    kernel_ms = ms1 - ms0
+    per_step_bytes = 79 * cells   # 79 B per cell-step, every stage of the time-step
+    achieved = per_step_bytes * steps / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_ens_step", "launches_per_step": 1.0,
+                "kernel_ms_per_step": kernel_ms / steps}
+    paths = [ens.path_stats(r) for r in range(len(reals))]
+    depth = [int(ens.levels(r).size) - 3 for r in range(len(reals))]
+    sched = ens.get_schedule().as_dict()
+    tune = ens.tune_log()
+    ens.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(1024, 29)
+    tiles = sum(t for t, _ in paths)
+    lvls = sum(l for _, l in paths)
+    line = {"metric": "ensemble cell-steps/s, FP64 (one persistent launch per step)",
+            "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": steps,
+            "warmup": args.warmup, "ms_per_step": secs * 1e3 / steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (per-realization mt19937 terrains; parameters from PCG64 seed 0)",
+            "config": {"workload": "ensemble of %d realizations, sides U{512..2048}, "
+                                   "K logU[1e-6,1e-4], U logU[5e-4,5e-3], m U[0.3,0.6], n=1, "
+                                   "dx=100, dt=1000, %d steps" % (len(reals), steps),
+                       "cells": cells, "schedule_final": sched,
+                       "realization_steps_tile_path": tiles,
+                       "realization_steps_level_path": lvls,
+                       "deepest_level_last_step": max(depth),
+                       "median_level_last_step": sorted(depth)[len(depth) // 2],
+                       "autotune_candidates_timed": len(tune)},
+            "roofline": roofline, "cpu_baseline": cpu}
+    if cpu:
+        line["speedup_vs_cpu_baseline"] = value / cpu["value"]
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -92,6 +162,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import fastscape_amd as fa
+
+    if args.workload == "ensemble":
+        bench_ensemble(args, fa)
+        return
+    args.steps = args.steps or 121
 
     if args.calibrate:
         ms = fa.calibrate_copy(1 << 30)
