@@ -60,6 +60,7 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, *COMMON, *DEVICE, "-shared", "-I" + INCLUDE, "-I" + CSRC,
           os.path.join(CSRC, "fastscape_hip.hip"), os.path.join(CSRC, "fs_ensemble.hip"),
+          os.path.join(CSRC, "fs_unet.hip"),
           os.path.join(CSRC, "fs_host.cpp"),
           "-o", tmp], verbose)
     os.replace(tmp, LIB)

@@ -9,9 +9,10 @@ from ._lib import (EXPORTED, FS_MAX_LEVELS, FS_NO_FLOW, KERNELS, LIB_PATH, PATH_
                    FsRealization, FsSchedule, calibrate_copy, generate_terrain, library, preset,
                    write_dem)
 from .ensemble import Ensemble, config4_realizations, realization, schedule
+from .unet import UNet
 
 __all__ = ["EXPORTED", "FS_MAX_LEVELS", "FS_NO_FLOW", "KERNELS", "LIB_PATH", "PATH_AUTO",
            "PATH_LEVELS", "TILE_HALO",
            "PRESET_BENCHMARK", "PRESET_REFERENCE", "TIMER_NAMES", "FastScape",
-           "FastScapeError", "FsParams", "FsRealization", "FsSchedule", "Ensemble",
+           "FastScapeError", "FsParams", "FsRealization", "FsSchedule", "Ensemble", "UNet",
            "config4_realizations", "realization", "schedule", "calibrate_copy", "generate_terrain", "library", "preset", "write_dem"]

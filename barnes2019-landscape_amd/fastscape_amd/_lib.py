@@ -39,6 +39,9 @@ EXPORTED = [
     "fs_ensemble_get_levels", "fs_ensemble_path_stats", "fs_ensemble_set_schedule",
     "fs_ensemble_get_schedule", "fs_ensemble_autotune", "fs_ensemble_tune_log",
     "fs_ensemble_stats", "fs_ensemble_synchronize",
+    "fs_unet_create", "fs_unet_destroy", "fs_unet_param_count", "fs_unet_layer_shapes",
+    "fs_unet_get_params", "fs_unet_set_params", "fs_unet_forward", "fs_unet_loss_grad",
+    "fs_unet_set_data", "fs_unet_train", "fs_unet_stats",
     "fs_write_dem", "fs_generate_terrain", "fs_strerror", "fs_last_error",
 ]
 
@@ -78,6 +81,7 @@ class FastScapeError(RuntimeError):
 
 _LIB = None
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int32)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _vp = ctypes.c_void_p
@@ -137,6 +141,20 @@ def _lib():
                                                 ctypes.c_int, _ip]),
         "fs_ensemble_stats": (ctypes.c_int, [_vp, _u64p, _dp]),
         "fs_ensemble_synchronize": (ctypes.c_int, [_vp]),
+        "fs_unet_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_vp)]),
+        "fs_unet_destroy": (ctypes.c_int, [_vp]),
+        "fs_unet_param_count": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+        "fs_unet_layer_shapes": (ctypes.c_int, [_vp, _ip, _ip, _ip, ctypes.c_int, _ip]),
+        "fs_unet_get_params": (ctypes.c_int, [_vp, _fp, ctypes.c_size_t]),
+        "fs_unet_set_params": (ctypes.c_int, [_vp, _fp, ctypes.c_size_t]),
+        "fs_unet_forward": (ctypes.c_int, [_vp, _fp, _fp, ctypes.c_int, _fp]),
+        "fs_unet_loss_grad": (ctypes.c_int, [_vp, _fp, _fp, _fp, ctypes.c_int, _fp, _fp,
+                                             ctypes.c_size_t]),
+        "fs_unet_set_data": (ctypes.c_int, [_vp, _fp, _fp, _fp, ctypes.c_int]),
+        "fs_unet_train": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                         ctypes.c_uint64, _fp]),
+        "fs_unet_stats": (ctypes.c_int, [_vp, _dp, _dp, _u64p]),
         "fs_write_dem": (ctypes.c_int, [ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int]),
         "fs_generate_terrain": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32]),
         "fs_strerror": (ctypes.c_char_p, [ctypes.c_int]),

@@ -255,6 +255,39 @@ int fs_ensemble_tune_log(fs_ensemble *e, fs_schedule *cand, double *ms_per_step,
 int fs_ensemble_stats(fs_ensemble *e, uint64_t *steps, double *step_ms);
 int fs_ensemble_synchronize(fs_ensemble *e);
 
+/* ---- learned proposal network ----------------------------------------------
+ * A 4-level convolutional U-Net (widths 32/64/128/160, ~1.4M parameters) in
+ * BF16 on the MFMA units with FP32 accumulation, trained with Adam on FP32
+ * master weights.  It maps (initial terrain, 3 normalised parameters) on an
+ * H x W grid (multiples of 32) to one output channel; the inversion driver
+ * uses it to rank candidate (K, U, m) sets so that only the best few are run
+ * through the exact time-step.  No reference counterpart.
+ * terrain: batch*H*W floats; params: batch*3 floats; outputs batch*H*W. */
+#define FS_UNET_IN_CHANNELS 8
+typedef struct fs_unet fs_unet;
+int fs_unet_create(int height, int width, int max_batch, uint64_t seed, int device,
+                   fs_unet **out);
+int fs_unet_destroy(fs_unet *u);
+int fs_unet_param_count(fs_unet *u, size_t *n);
+/* Parameter vector: per conv layer W[tap][out][in] then bias[out], then the
+ * head's 32 weights and its bias.  Layer i has cin[i] -> cout[i] channels at
+ * resolution (H >> level[i]) x (W >> level[i]). */
+int fs_unet_layer_shapes(fs_unet *u, int *cin, int *cout, int *level, int capacity, int *count);
+int fs_unet_get_params(fs_unet *u, float *p, size_t n);
+int fs_unet_set_params(fs_unet *u, const float *p, size_t n);
+int fs_unet_forward(fs_unet *u, const float *terrain, const float *params, int batch, float *y);
+/* loss = mean((y - target)^2) over batch*H*W, and its gradient w.r.t. every
+ * parameter (grad may be NULL).  batch <= max_batch. */
+int fs_unet_loss_grad(fs_unet *u, const float *terrain, const float *params, const float *target,
+                      int batch, float *loss, float *grad, size_t n);
+/* Training set kept in device memory; fs_unet_train runs `epochs` passes of
+ * shuffled minibatches, one Adam step each, learning rate decayed by a
+ * cosine from lr to lr/20; epoch_loss[epochs] receives mean losses. */
+int fs_unet_set_data(fs_unet *u, const float *terrain, const float *params, const float *target,
+                     int count);
+int fs_unet_train(fs_unet *u, int epochs, int batch, float lr, uint64_t seed, float *epoch_loss);
+int fs_unet_stats(fs_unet *u, double *train_ms, double *flops_per_sample, uint64_t *samples);
+
 /* ---- host helpers ---------------------------------------------------------- */
 /* PrintDEM (RB:20-40): ArcGIS ASCII grid of the inner (W-2)x(H-2) cells,
  * byte-identical to the reference writer. */

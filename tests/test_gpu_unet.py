@@ -1,0 +1,108 @@
+"""GPU checks of the learned-proposal network's hand-written kernels
+(BF16 MFMA forward, backward, Adam) against the CPU reference
+tests/unet_ref.py, which the finite-difference and autograd tests pin.
+
+The reference is run with BF16 rounding at the points where the device
+stores BF16 (input, weights, activations, activation gradients).  The
+device accumulates in FP32 in its own order, so single BF16 roundings can
+differ; the bars are therefore relative: forward outputs within 3% of the
+output range, every layer's gradient within 5% relative L2 error and a
+cosine similarity above 0.995.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import fastscape_amd as fa
+import unet_ref as ur
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print("[gpu] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def sample(seed, B, H, W):
+    rng = np.random.default_rng(seed)
+    terrain = rng.random((B, H, W)).astype(np.float32)
+    params = rng.uniform(-1, 1, (B, 3)).astype(np.float32)
+    target = rng.normal(0, 1, (B, H, W)).astype(np.float32)
+    return terrain, params, target
+
+
+def perturbed_params(net, seed):
+    p = net.get_params().astype(np.float64)
+    p += np.random.default_rng(seed).normal(0, 0.02, p.size)
+    return p.astype(np.float32)
+
+
+def test_layout_matches_reference():
+    with fa.UNet(32, 32, max_batch=2) as net:
+        assert net.layers() == ur.layer_shapes()
+        assert net.nparam == ur.param_count()
+
+
+@pytest.mark.parametrize("H,W,B", [(32, 64, 3), (64, 32, 2)])
+def test_forward_matches_reference(H, W, B):
+    note("unet forward %dx%d batch %d" % (H, W, B))
+    terrain, params, _ = sample(H + W, B, H, W)
+    with fa.UNet(H, W, max_batch=2, seed=11) as net:
+        p = perturbed_params(net, 12)
+        net.set_params(p)
+        y = net.forward(terrain, params)          # also exercises batching over max_batch
+    ref = ur.Net(p.astype(np.float64), bf16=True).forward(terrain.astype(np.float64),
+                                                          params.astype(np.float64))
+    span = np.abs(ref).max()
+    err = np.abs(y - ref).max()
+    note("forward max err %.3e of range %.3e" % (err, span))
+    assert err <= 0.03 * span + 1e-3
+    assert np.corrcoef(y.ravel(), ref.ravel())[0, 1] > 0.999
+
+
+def test_gradients_match_reference():
+    H, W, B = 32, 32, 2
+    terrain, params, target = sample(5, B, H, W)
+    with fa.UNet(H, W, max_batch=B, seed=21) as net:
+        p = perturbed_params(net, 22)
+        net.set_params(p)
+        loss, g = net.loss_grad(terrain, params, target)
+    rl, rg = ur.Net(p.astype(np.float64), bf16=True).loss_grad(
+        terrain.astype(np.float64), params.astype(np.float64), target.astype(np.float64))
+    note("loss gpu %.6f ref %.6f" % (loss, rl))
+    assert abs(loss - rl) <= 1e-2 * abs(rl)
+    off = 0
+    for i, (ci, co, lv) in enumerate(ur.layer_shapes()):
+        for what, n in (("W", 9 * ci * co), ("b", co)):
+            a, b = g[off:off + n].astype(np.float64), rg[off:off + n]
+            off += n
+            rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+            cos = a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30)
+            assert rel < 0.05 and cos > 0.995, (i, what, rel, cos)
+    a, b = g[off:].astype(np.float64), rg[off:]
+    assert np.linalg.norm(a - b) <= 0.02 * np.linalg.norm(b)
+
+
+def test_training_reduces_loss():
+    """A smooth function of the inputs is learned: the loss falls tenfold."""
+    H = W = 32
+    n = 64
+    rng = np.random.default_rng(9)
+    terrain = rng.random((n, H, W)).astype(np.float32)
+    params = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    yy, xx = np.mgrid[0:H, 0:W] / H
+    target = (params[:, 0, None, None] * np.sin(3 * xx) + params[:, 1, None, None] * yy +
+              0.5 * params[:, 2, None, None] + 0.3 * terrain).astype(np.float32)
+    with fa.UNet(H, W, max_batch=16, seed=1) as net:
+        net.set_data(terrain, params, target)
+        losses = net.train(epochs=40, batch=16, lr=2e-3, seed=3)
+        note("epoch losses %s" % np.array2string(losses[::8], precision=4))
+        assert np.all(np.isfinite(losses))
+        assert losses[-1] < 0.1 * losses[0]
+        pred = net.forward(terrain[:8], params[:8])
+        mse = float(np.mean((pred - target[:8]) ** 2))
+        assert mse < 0.2 * float(np.mean((target[:8] - target[:8].mean()) ** 2))
+        st = net.stats()
+        assert st["samples"] == 40 * 64 and st["train_ms"] > 0
