@@ -363,15 +363,40 @@ __global__ __launch_bounds__(256) void k_upcat_grad(const uint16_t *__restrict__
 }
 
 // network input: [terrain, p0, p1, p2, 0, 0, 0, 0] per pixel
+// (terrain_stride 0: one terrain shared by the batch)
 __global__ __launch_bounds__(256) void k_pack_input(const float *__restrict__ terrain,
                                                     const float *__restrict__ params,
-                                                    uint16_t *__restrict__ x, int B, long HW) {
+                                                    uint16_t *__restrict__ x, int B, long HW,
+                                                    long terrain_stride) {
   for (long i = blockIdx.x * 256l + threadIdx.x; i < B * HW; i += gridDim.x * 256l) {
     const long b = i / HW;
-    uint16_t o[8] = {f2bf(terrain[i]), f2bf(params[3 * b]), f2bf(params[3 * b + 1]),
+    uint16_t o[8] = {f2bf(terrain[b * terrain_stride + i % HW]), f2bf(params[3 * b]),
+                     f2bf(params[3 * b + 1]),
                      f2bf(params[3 * b + 2]), 0, 0, 0, 0};
     *reinterpret_cast<uint4 *>(x + i * 8) = *reinterpret_cast<const uint4 *>(o);
   }
+}
+
+// sq[b] += sum over the interior (rings 0, 1 excluded) of (yhat - t)^2 for
+// the batch's samples, against one target t of H x W
+__global__ __launch_bounds__(256) void k_sq_misfit(const float *__restrict__ yhat,
+                                                   const float *__restrict__ t,
+                                                   float *__restrict__ sq, int H, int W) {
+  __shared__ float red;
+  if (threadIdx.x == 0) red = 0.f;
+  __syncthreads();
+  const long HW = static_cast<long>(H) * W;
+  const int b = blockIdx.y;
+  float s = 0.f;
+  for (long i = blockIdx.x * 256l + threadIdx.x; i < HW; i += gridDim.x * 256l) {
+    const int x = static_cast<int>(i % W), y = static_cast<int>(i / W);
+    if (x < 2 || y < 2 || x >= W - 2 || y >= H - 2) continue;
+    const float d = yhat[b * HW + i] - t[i];
+    s += d * d;
+  }
+  atomicAdd(&red, s);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&sq[b], red);
 }
 
 // head: yhat[p] = hb + sum_c hw[c] * f[p][c], C = 32
@@ -602,10 +627,11 @@ int pack_weights(fs_unet *u) {
 int enc(int L, int k) { return 2 * L + k; }
 int dec(int L, int k) { return 8 + 2 * (2 - L) + k; }
 
-int forward(fs_unet *u, const float *terrain_d, const float *params_d, int B) {
+int forward(fs_unet *u, const float *terrain_d, const float *params_d, int B,
+            bool shared_terrain = false) {
   const long HW = static_cast<long>(u->H) * u->W;
   hipLaunchKernelGGL(k_pack_input, dim3(grid_for(B * HW)), dim3(256), 0, u->stream, terrain_d,
-                     params_d, u->x0.d, B, HW);
+                     params_d, u->x0.d, B, HW, shared_terrain ? 0l : HW);
   FS_TRY(check_launch_u());
   const Tensor *in = &u->x0;
   for (int L = 0; L < kLevels; L++) {
@@ -1003,6 +1029,46 @@ int fs_unet_train(fs_unet *u, int epochs, int batch, float lr, uint64_t seed, fl
   }
   u->train_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   u->samples_trained += static_cast<uint64_t>(total) * batch;
+  return FS_OK;
+}
+
+int fs_unet_rank(fs_unet *u, const float *terrain, const float *params, int count,
+                 const float *target, float *misfit) {
+  FS_TRY(check_unet(u));
+  if (!terrain || !params || !target || !misfit || count < 1) return FS_ERR_INVALID_ARGUMENT;
+  const size_t HW = static_cast<size_t>(u->H) * u->W;
+  float *p_d = nullptr, *sq_d = nullptr;
+  FS_TRY(fs::dev_alloc(&p_d, 3ul * count));
+  struct Free {
+    float *a, *b;
+    ~Free() {
+      if (a) (void)hipFree(a);
+      if (b) (void)hipFree(b);
+    }
+  } fr{p_d, nullptr};
+  FS_TRY(fs::dev_alloc(&sq_d, static_cast<size_t>(count)));
+  fr.b = sq_d;
+  FS_HIP_TRY(hipMemcpyAsync(p_d, params, 3ul * count * sizeof(float), hipMemcpyHostToDevice,
+                            u->stream));
+  // terrain in u->terrain[0], target in u->tgt[0]
+  FS_HIP_TRY(hipMemcpyAsync(u->terrain, terrain, HW * sizeof(float), hipMemcpyHostToDevice,
+                            u->stream));
+  FS_HIP_TRY(hipMemcpyAsync(u->tgt, target, HW * sizeof(float), hipMemcpyHostToDevice, u->stream));
+  hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(count)), dim3(256), 0, u->stream, sq_d,
+                     static_cast<long>(count), 0.f);
+  for (int b0 = 0; b0 < count; b0 += u->max_batch) {
+    const int B = std::min(u->max_batch, count - b0);
+    FS_TRY(forward(u, u->terrain, p_d + 3l * b0, B, true));
+    hipLaunchKernelGGL(k_sq_misfit, dim3(static_cast<unsigned>(std::min<size_t>((HW + 255) / 256, 64)), B),
+                       dim3(256), 0, u->stream, u->yhat, u->tgt, sq_d + b0, u->H, u->W);
+    FS_TRY(check_launch_u());
+  }
+  std::vector<float> sq(count);
+  FS_HIP_TRY(hipMemcpyAsync(sq.data(), sq_d, count * sizeof(float), hipMemcpyDeviceToHost,
+                            u->stream));
+  FS_HIP_TRY(hipStreamSynchronize(u->stream));
+  const double ninterior = static_cast<double>(u->H - 4) * (u->W - 4);
+  for (int i = 0; i < count; i++) misfit[i] = static_cast<float>(std::sqrt(sq[i] / ninterior));
   return FS_OK;
 }
 

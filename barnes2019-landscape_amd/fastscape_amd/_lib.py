@@ -41,7 +41,7 @@ EXPORTED = [
     "fs_ensemble_stats", "fs_ensemble_synchronize",
     "fs_unet_create", "fs_unet_destroy", "fs_unet_param_count", "fs_unet_layer_shapes",
     "fs_unet_get_params", "fs_unet_set_params", "fs_unet_forward", "fs_unet_loss_grad",
-    "fs_unet_set_data", "fs_unet_train", "fs_unet_stats",
+    "fs_unet_set_data", "fs_unet_train", "fs_unet_rank", "fs_unet_stats",
     "fs_write_dem", "fs_generate_terrain", "fs_strerror", "fs_last_error",
 ]
 
@@ -154,6 +154,7 @@ def _lib():
         "fs_unet_set_data": (ctypes.c_int, [_vp, _fp, _fp, _fp, ctypes.c_int]),
         "fs_unet_train": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                          ctypes.c_uint64, _fp]),
+        "fs_unet_rank": (ctypes.c_int, [_vp, _fp, _fp, ctypes.c_int, _fp, _fp]),
         "fs_unet_stats": (ctypes.c_int, [_vp, _dp, _dp, _u64p]),
         "fs_write_dem": (ctypes.c_int, [ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int]),
         "fs_generate_terrain": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32]),

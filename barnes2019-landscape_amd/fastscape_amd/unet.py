@@ -96,6 +96,15 @@ class UNet:
                                     _ptr(losses, _f)), "fs_unet_train")
         return losses
 
+    def rank(self, terrain, params, target):
+        """RMS interior misfit of each params row's output (one shared terrain)
+        against target; outputs stay on the device."""
+        terrain, params, target = _f32(terrain), _f32(params), _f32(target)
+        mis = np.empty(params.shape[0], dtype=np.float32)
+        _check(_lib().fs_unet_rank(self._u, _ptr(terrain, _f), _ptr(params, _f), params.shape[0],
+                                   _ptr(target, _f), _ptr(mis, _f)), "fs_unet_rank")
+        return mis
+
     def stats(self):
         ms, fl = ctypes.c_double(0), ctypes.c_double(0)
         n = ctypes.c_uint64(0)

@@ -10,6 +10,7 @@ timed region.
 
     python bench.py [--gpus 1] [--steps 121] [--warmup 5] [--size 10000]
     python bench.py --workload ensemble [--steps 500]     # 256 realizations
+    python bench.py --workload inversion                  # learned proposal vs exact-only
 
 
 Prints one JSON line.  `value` is cell-steps/s over the timed steps;
@@ -49,9 +50,14 @@ KERNEL_BYTES = {
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["grid", "ensemble"], default="grid",
+    ap.add_argument("--workload", choices=["grid", "ensemble", "inversion"], default="grid",
                     help="grid: one 10000^2 grid (default); ensemble: 256 realizations of "
-                         "sides 512..2048 with sampled K, U, m")
+                         "sides 512..2048 with sampled K, U, m; inversion: learned proposal "
+                         "vs exact-only search at 256^2, 2000 steps")
+    ap.add_argument("--train-runs", type=int, default=20480)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--candidates-axis", type=int, default=16)
+    ap.add_argument("--top-k", type=int, default=32)
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (grid: 121; ensemble: 500)")
     ap.add_argument("--warmup", type=int, default=5)
@@ -154,6 +160,96 @@ def bench_ensemble(args, fa):
     print(json.dumps(line))
 
 
+def bench_inversion(args, fa):
+    """Parameter inversion on 256 x 256 grids, 2000 time-steps per exact run:
+    a training set of --train-runs exact runs (parameters from the ensemble
+    workload's distribution, terrain seeds 1..n), the U-Net trained on it,
+    then a hidden-parameter target (terrain seed 900001) searched over a
+    16^3 candidate grid by (a) running every candidate exactly and (b)
+    ranking with the network and running the top 32 exactly.  `value` is
+    (b)'s wall time; the exact solver judges both results."""
+    import numpy as np
+    from fastscape_amd import inversion as inv
+    side, nstep = 256, 2000
+
+    def log(msg):
+        print("[inversion] " + msg, file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    terrain, pn, t, params, gen_s = inv.make_training_set(args.train_runs, side, nstep, seed0=1,
+                                                          master_seed=0, log=log)
+    gen_wall = time.perf_counter() - t0
+    log("training set: %d exact runs, %.1f s in the solver, %.1f s wall"
+        % (args.train_runs, gen_s, gen_wall))
+    prop = inv.Proposal(side, max_batch=32, seed=0)
+    hist = prop.train(terrain, pn, t, epochs=args.epochs, batch=32, lr=1e-3, seed=1)
+    st = prop.net.stats()
+    train_tflops = 3 * st["flops_per_sample"] * st["samples"] / (st["train_ms"] * 1e-3) / 1e12
+    log("trained %d epochs in %.1f s (%.1f TFLOP/s), losses %s"
+        % (args.epochs, st["train_ms"] / 1e3, train_tflops, np.array2string(hist, precision=4)))
+    del terrain, t
+    hidden = np.array([[2.3e-5, 1.7e-3, 0.44]])
+    seed = 900001
+    target = {}
+    inv.exact_runs(hidden, seed, side, nstep, on_result=lambda i, h: target.update(h=h))
+    tt = inv.transform(target["h"])
+    cands = inv.candidate_grid(args.candidates_axis)
+    best_e, mis_e, t_exact = inv.exact_only_search(tt, seed, cands, side, nstep)
+    log("exact-only: %d candidates in %.2f s, best %d misfit %.5f"
+        % (len(cands), t_exact, best_e, mis_e[best_e]))
+    best_p, mis_p, top, pred, t_prop = inv.proposal_search(prop, tt, seed, cands, side, nstep,
+                                                           top_k=args.top_k)
+    log("proposal: top %d in %.2f s, best %d misfit %.5f"
+        % (args.top_k, t_prop, best_p, mis_p.min()))
+    # inference alone, for the MFMA roofline
+    terrain1 = fa.generate_terrain(side, side, seed).reshape(side, side).astype(np.float32)
+    pn_c = inv.normalise(cands[:, 0], cands[:, 1], cands[:, 2])
+    prop.rank(terrain1, pn_c[:64], tt)
+    ti = time.perf_counter()
+    prop.rank(terrain1, pn_c, tt)
+    t_inf = time.perf_counter() - ti
+    inf_tflops = st["flops_per_sample"] * len(cands) / t_inf / 1e12
+    rel = inv.relative_param_error(cands[best_p], cands[best_e])
+    rank_corr = float(np.corrcoef(np.argsort(np.argsort(pred)),
+                                  np.argsort(np.argsort(mis_e)))[0, 1])
+    prop.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        one = cpu_baseline(side, nstep - 1)
+        per_run = side * side * nstep / one["value"]
+        cpu = {"value": per_run * len(cands), "unit": "s (exact-only search on the CPU)",
+               "cores": one["cores"], "kind": one["kind"],
+               "sample": "one %dx%d run(%d) of the benchmark preset, %.2f s, times %d candidates"
+                         % (side, side, nstep - 1, per_run, len(cands))}
+    line = {"metric": "exact-vs-proposal inversion wall time", "value": t_prop, "unit": "s",
+            "n_gpus": 1, "steps": nstep, "warmup": 0, "ms_per_step": None,
+            "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+            "dtype": "f64 solver, bf16 network (fp32 accumulate)",
+            "data": "synthetic (mt19937 terrains; parameters from PCG64 seed 0)",
+            "config": {"workload": "inversion: %d^3 candidates, top %d exact reruns, %dx%d, "
+                                   "%d steps, network trained on %d exact runs x %d epochs"
+                                   % (args.candidates_axis, args.top_k, side, side, nstep,
+                                      args.train_runs, args.epochs),
+                       "hidden_params": hidden[0].tolist(), "terrain_seed": seed},
+            "exact_only_seconds": t_exact, "speedup_vs_exact_only": t_exact / t_prop,
+            "result": {"exact_only_best": cands[best_e].tolist(),
+                       "proposal_best": cands[best_p].tolist(),
+                       "same_candidate": bool(best_p == best_e),
+                       "relative_param_error": rel.tolist(),
+                       "misfit_exact_only": float(mis_e[best_e]),
+                       "misfit_proposal": float(mis_p.min()),
+                       "rank_correlation_predicted_vs_exact": rank_corr},
+            "training": {"runs": args.train_runs, "solver_seconds": gen_s,
+                         "train_seconds": st["train_ms"] / 1e3, "epochs": args.epochs,
+                         "final_loss": float(hist[-1]), "tflops": train_tflops},
+            "roofline": {"bound": "mfma", "achieved": round(inf_tflops, 2), "peak": 2500.0,
+                         "unit": "TFLOP/s", "frac": round(inf_tflops / 2500.0, 4),
+                         "traffic": None, "kernel": "U-Net inference (k_conv3x3)",
+                         "inference_seconds": t_inf},
+            "cpu_baseline": cpu}
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -165,6 +261,9 @@ def main():
 
     if args.workload == "ensemble":
         bench_ensemble(args, fa)
+        return
+    if args.workload == "inversion":
+        bench_inversion(args, fa)
         return
     args.steps = args.steps or 121
 

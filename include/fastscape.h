@@ -286,6 +286,11 @@ int fs_unet_loss_grad(fs_unet *u, const float *terrain, const float *params, con
 int fs_unet_set_data(fs_unet *u, const float *terrain, const float *params, const float *target,
                      int count);
 int fs_unet_train(fs_unet *u, int epochs, int batch, float lr, uint64_t seed, float *epoch_loss);
+/* Misfit of `count` parameter sets sharing one terrain against one target:
+ * misfit[i] = RMS over the interior cells (rings 0 and 1 excluded) of
+ * output_i - target.  Outputs stay on the device. */
+int fs_unet_rank(fs_unet *u, const float *terrain, const float *params, int count,
+                 const float *target, float *misfit);
 int fs_unet_stats(fs_unet *u, double *train_ms, double *flops_per_sample, uint64_t *samples);
 
 /* ---- host helpers ---------------------------------------------------------- */
