@@ -47,10 +47,22 @@ KERNEL_BYTES = {
 }
 
 
+# HBM bytes per cell actually moved by one launch (FETCH_SIZE x 2.0 + WRITE_SIZE,
+# calibrated against a 1 GiB copy) at 10000 x 10000 with the benchmark preset:
+# profiles/tile_path_10000x10000_hbm_traffic.md.  Reported as `traffic` for
+# that workload only.
+MEASURED_TRAFFIC_PER_CELL = {
+    "k_tile_erode": 29.03,
+    "k_tile_acc": 6.18,
+    "k_receivers": 10.71,
+}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["grid", "ensemble", "inversion"], default="grid",
+    ap.add_argument("--workload", choices=["grid", "ensemble", "inversion", "unet"],
+                    default="grid",
                     help="grid: one 10000^2 grid (default); ensemble: 256 realizations of "
                          "sides 512..2048 with sampled K, U, m; inversion: learned proposal "
                          "vs exact-only search at 256^2, 2000 steps")
@@ -250,6 +262,47 @@ def bench_inversion(args, fa):
     print(json.dumps(line))
 
 
+def bench_unet(args, fa):
+    """Throughput of the learned proposal's network alone: BF16 MFMA forward
+    passes (inference) and training steps (forward + backward + Adam) on
+    synthetic 256 x 256 inputs, 32 samples per batch."""
+    import numpy as np
+    side, batch, n = 256, 32, 256
+    rng = np.random.default_rng(0)
+    terrain = rng.random((n, side, side), dtype=np.float32)
+    params = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    target = rng.normal(0, 1, (n, side, side)).astype(np.float32)
+    with fa.UNet(side, side, max_batch=batch, seed=0) as net:
+        fl = net.stats()["flops_per_sample"]
+        net.forward(terrain[:batch], params[:batch])
+        t0 = time.perf_counter()
+        net.forward(terrain, params)
+        t_inf = time.perf_counter() - t0
+        net.set_data(terrain, params, target)
+        net.train(1, batch)
+        st0 = net.stats()
+        net.train(args.epochs, batch)
+        st1 = net.stats()
+    inf = fl * n / t_inf / 1e12
+    tr_ms = st1["train_ms"] - st0["train_ms"]
+    tr_samples = st1["samples"] - st0["samples"]
+    tr = 3 * fl * tr_samples / (tr_ms * 1e-3) / 1e12
+    line = {"metric": "U-Net inference TFLOP/s (BF16 MFMA, FP32 accumulate)", "value": inf,
+            "unit": "TFLOP/s", "n_gpus": 1, "steps": n // batch, "warmup": 1,
+            "ms_per_step": t_inf * 1e3 / (n // batch), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+            "data": "synthetic (uniform terrain, uniform parameters, normal targets)",
+            "config": {"workload": "4-level U-Net 32/64/128/160, %dx%d, batch %d, %d samples"
+                                   % (side, side, batch, n),
+                       "gflop_per_sample_forward": fl / 1e9,
+                       "training_tflops": tr, "training_samples": tr_samples},
+            "roofline": {"bound": "mfma", "achieved": round(inf, 2), "peak": 2500.0,
+                         "unit": "TFLOP/s", "frac": round(inf / 2500.0, 4), "traffic": None,
+                         "kernel": "U-Net forward (k_conv3x3)"},
+            "cpu_baseline": None}
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -259,18 +312,20 @@ def main():
         sys.exit(2)
     import fastscape_amd as fa
 
+    if args.calibrate:
+        ms = fa.calibrate_copy(1 << 30)
+        print("calibration copy: 1 GiB in %.3f ms (%.1f GB/s read+write)"
+              % (ms, 2 * (1 << 30) / ms / 1e6), file=sys.stderr)
     if args.workload == "ensemble":
         bench_ensemble(args, fa)
         return
     if args.workload == "inversion":
         bench_inversion(args, fa)
         return
+    if args.workload == "unet":
+        bench_unet(args, fa)
+        return
     args.steps = args.steps or 121
-
-    if args.calibrate:
-        ms = fa.calibrate_copy(1 << 30)
-        print("calibration copy: 1 GiB in %.3f ms (%.1f GB/s read+write)"
-              % (ms, 2 * (1 << 30) / ms / 1e6), file=sys.stderr)
     W = H = args.size
     N = W * H
     m = fa.FastScape(W, H, fa.preset(fa.PRESET_BENCHMARK), engine_seed=0, device=0)
@@ -297,8 +352,11 @@ def main():
     ms, launches = stats[dom]
     per_step_bytes = KERNEL_BYTES.get(dom, 0) * units.get(dom, N)
     achieved = per_step_bytes * args.profile_steps / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    traffic = None
+    if args.size == 10000 and dom in MEASURED_TRAFFIC_PER_CELL:
+        traffic = MEASURED_TRAFFIC_PER_CELL[dom] * N
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": dom, "launches_per_step": launches / args.profile_steps,
                 "kernel_ms_per_step": ms / args.profile_steps}
     m.close()
