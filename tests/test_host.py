@@ -63,6 +63,10 @@ def test_terrain_generator_matches_reference(fx):
 def test_terrain_generator_matches_oracle_large(oracle):
     np.testing.assert_array_equal(fa.generate_terrain(1000, 700, 0), oracle.init_terrain(1000, 700, 0))
     np.testing.assert_array_equal(fa.generate_terrain(333, 77, 99), oracle.init_terrain(333, 77, 99))
+    # the seeds the ensemble and inversion workloads use
+    for seed in (1, 255, 900001):
+        np.testing.assert_array_equal(fa.generate_terrain(64, 96, seed),
+                                      oracle.init_terrain(64, 96, seed))
 
 
 def test_dem_writer_byte_identical(oracle, tmp_path):
