@@ -710,14 +710,45 @@ __device__ __forceinline__ void load_region(const uint8_t *__restrict__ rcv,
 // number of region cells draining through it -- its whole upstream area when
 // every level is at most kHalo.  A tile cell deeper than that (or whose chain
 // leaves the region) raises hdr[3]; the host then runs the level path.
+// Halo cells the erosion pass solves per tile at most: those on the receiver
+// chains of the tile's cells (mean 60-66, at most 92 per tile of 5120 over
+// 120 steps at the benchmark preset).
+constexpr int kHaloCap = 256;
+
+__device__ __forceinline__ bool in_tile(int lx, int ly) {
+  return lx >= kHalo && lx < kHalo + kTile && ly >= kHalo && ly < kHalo + kTile;
+}
+
+// Mark (bitmask over region cells) the halo cells on the receiver chains of
+// the tile's cells: walks start at tile cells whose receiver leaves the tile
+// and stop at an outlet, the region edge, a tile cell, an already marked cell
+// or after kHalo hops.  mark must be zero and r_s loaded before the call.
+template <int BLOCK>
+__device__ __forceinline__ void mark_halo_chains(const uint8_t *r_s, uint32_t *mark) {
+  for (int it = 0; it < kTile * kTile / BLOCK; it++) {
+    const int j = it * BLOCK + threadIdx.x;
+    int lx = kHalo + j % kTile, ly = kHalo + j / kTile;
+    int r = r_s[ly * kReg + lx];
+    for (int hop = 0; hop <= kHalo && r < 8; hop++) {
+      if (!region_step(r, lx, ly) || in_tile(lx, ly)) break;
+      const int c = ly * kReg + lx;
+      const uint32_t bit = 1u << (c & 31);
+      if (atomicOr(&mark[c >> 5], bit) & bit) break;   // marked by another walk
+      r = r_s[c];
+    }
+  }
+}
+
 struct AccShared {
   uint8_t r_s[kRegCells];
   uint32_t c_s[kRegCells];
-  int over;
+  uint32_t mark[kRegCells / 32];
+  int over, nmarked;
   int hist[kHalo + 1];
 };
 
-// Returns nonzero (block-uniform) when a tile cell is deeper than kHalo.
+// Returns nonzero (block-uniform) when a tile cell is deeper than kHalo or
+// the tile's chains need more than kHaloCap halo cells in the erosion pass.
 // With level_hist, also adds the tile cells' level histogram to it.
 template <int BLOCK>
 __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__restrict__ rcv,
@@ -728,8 +759,9 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
   uint32_t *c_s = sh.c_s;
   const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
   const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
-  if (threadIdx.x == 0) sh.over = 0;
+  if (threadIdx.x == 0) sh.over = sh.nmarked = 0;
   if (threadIdx.x < kHalo + 1) sh.hist[threadIdx.x] = 0;
+  if (threadIdx.x < kRegCells / 32) sh.mark[threadIdx.x] = 0;
   for (int it = 0; it < kRegCells / BLOCK; it++) c_s[it * BLOCK + threadIdx.x] = 0;
   load_region<BLOCK, false>(rcv, nullptr, nullptr, nullptr, r_s, nullptr, nullptr, nullptr,
                             rx0, ry0, W, H);
@@ -783,6 +815,14 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
     }
   }
   if (over) sh.over = 1;
+  mark_halo_chains<BLOCK>(r_s, sh.mark);
+  __syncthreads();
+  if (threadIdx.x < kRegCells / 32) {
+    const int c = __popc(sh.mark[threadIdx.x]);
+    if (c) atomicAdd(&sh.nmarked, c);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && sh.nmarked > kHaloCap) sh.over = 1;
   __syncthreads();
   if (level_hist && threadIdx.x < kHalo + 1 && sh.hist[threadIdx.x])
     atomicAdd(&level_hist[threadIdx.x], sh.hist[threadIdx.x]);
@@ -807,29 +847,48 @@ __global__ __launch_bounds__(kAccBlock) void k_tile_acc(const uint8_t *__restric
 }
 
 // Pass 2: uplift and erosion (AddUplift RB:285-296, Erode RB:306-328) of the
-// tile's cells.  Levels come from pass 1.  Region cells are bucketed by
-// level into a list whose level segments are padded to whole 64-entry
-// chunks; each wave takes chunks in increasing order and each lane solves
-// its cell with the reference's Newton iteration on LDS-resident elevations
-// as soon as its receiver (one level lower, hence in an earlier chunk) is
-// marked done.  With one level per chunk no lane waits on a lane of its own
-// wave, and the lowest chunk in progress never waits on an unfinished one,
-// so the wavefront always advances; the wait is bounded all the same and
-// reported in hdr[2].  The tile cells' receiver chains lie in the region
-// because every level is at most kHalo.  Reads h_old, writes h_new:
-// neighbouring tiles read this tile's cells as their halo.
+// tile's cells.  Levels and counts come from pass 1.  Only the tile's cells
+// and the halo cells on their receiver chains are solved: mark_halo_chains
+// finds the latter (at most kHaloCap, checked in pass 1; the chains lie in
+// the region because every level is at most kHalo).  Cells live in slots --
+// tile cells at their tile index, marked halo cells after them in region
+// order, a halo slot being the rank of its mark bit -- so LDS holds 79 KB and
+// two tiles share a CU.  The needed cells are bucketed by level into a list
+// whose level segments are padded to whole 64-entry chunks; each wave takes
+// chunks in increasing order and each lane solves its cell with the
+// reference's Newton iteration as soon as its receiver (one level lower,
+// hence in an earlier chunk) is marked done.  With one level per chunk no
+// lane waits on a lane of its own wave, and the lowest chunk in progress
+// never waits on an unfinished one, so the wavefront always advances; the
+// wait is bounded all the same and reported in hdr[2].  Reads h_old, writes
+// h_new: neighbouring tiles read this tile's cells as their halo.  Callers
+// that run several tiles per workgroup put a barrier between calls.
 constexpr uint8_t kDone = 0xFD;
-constexpr int kListCap = kRegCells + (kHalo + 1) * (kWave - 1);
+constexpr int kTileCells = kTile * kTile;
+constexpr int kSlots = kTileCells + kHaloCap;
+constexpr int kListCap = kSlots + (kHalo + 1) * (kWave - 1);
+constexpr int kEroFactCache = 64;             // erosion factors cached in LDS
 struct EroShared {
-  double hs[kRegCells];
-  double ft_s[2 * kFactCache];
-  uint32_t c_s[kRegCells];
+  double hs[kSlots];
+  uint32_t c_s[kSlots];
+  double ft_s[2 * kEroFactCache];
   uint16_t list[kListCap];
+  uint16_t hreg[kHaloCap];      // region index of each halo slot
+  uint16_t pre[kRegCells / 32]; // marks before each mark word
   uint8_t r_s[kRegCells];
-  uint8_t d_s[kRegCells];       // level, then kDone once solved
+  uint8_t d_s[kSlots];          // level, then kDone once solved
+  uint32_t mark[kRegCells / 32];
+  uint32_t scan[kEroBlock / kWave + 1];
   int wcount[kHalo + 1][kEroBlock / kWave];
   int ltotal[kHalo + 1], loff[kHalo + 2];
 };
+
+// slot of region cell c (a tile cell, or a marked halo cell)
+__device__ __forceinline__ int ero_slot(const EroShared &sh, int c) {
+  const int lx = c % kReg, ly = c / kReg;
+  if (in_tile(lx, ly)) return (ly - kHalo) * kTile + (lx - kHalo);
+  return kTileCells + sh.pre[c >> 5] + __popc(sh.mark[c >> 5] & ((1u << (c & 31)) - 1u));
+}
 
 template <int NMODE>
 __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__restrict__ h_old,
@@ -843,6 +902,11 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
   constexpr int kWaves = kEroBlock / kWave;
   constexpr int kKeys = kHalo + 1;
   constexpr int kPer = kRegCells / kEroBlock;
+  constexpr int kTilePer = kTileCells / kEroBlock;
+  constexpr int kSlotPer = (kSlots + kEroBlock - 1) / kEroBlock;
+  static_assert(kKeys <= 32, "level keys must fit the 5-bit match");
+  static_assert(kRegCells / 32 <= kEroBlock && 2 * kEroFactCache <= kEroBlock, "block shape");
+  static_assert(kEroFactCache <= kFactCache, "factor table holds the cached counts");
   double *hs = sh.hs, *ft_s = sh.ft_s;
   uint32_t *c_s = sh.c_s;
   uint16_t *list = sh.list;
@@ -853,15 +917,129 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
   const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   for (int i = threadIdx.x; i < kKeys * kWaves; i += kEroBlock) (&wcount[0][0])[i] = 0;
-  if (threadIdx.x < 2 * kFactCache) ft_s[threadIdx.x] = ftab[threadIdx.x];
-  load_region<kEroBlock, true>(rcv, h_old, lvl, cnt, r_s, hs, d_s, c_s, rx0, ry0, W, H);
-  __syncthreads();
-  // bucket the region cells by level: rank within (level, wave), then bases
-  int rank[kPer];
+  if (threadIdx.x < kRegCells / 32) sh.mark[threadIdx.x] = 0;
+  if (threadIdx.x < 2 * kEroFactCache) ft_s[threadIdx.x] = ftab[threadIdx.x];
+  // receivers of the whole region; elevation, count and level of the tile's
+  // cells
+  {
+    uint8_t rv[kPer], tl[kTilePer];
+    double th[kTilePer];
+    uint32_t tc[kTilePer];
 #pragma unroll
-  for (int q = 0; q < kPer; q++) {
-    const int d = d_s[q * kEroBlock + threadIdx.x];
-    rank[q] = d < kKeys ? atomicAdd(&wcount[d][wave], 1) : -1;
+    for (int q = 0; q < kPer; q++) {
+      const int i = q * kEroBlock + threadIdx.x;
+      const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
+      const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
+      rv[q] = in ? rcv[static_cast<size_t>(gy) * W + gx] : kOutside;
+    }
+#pragma unroll
+    for (int j = 0; j < kTilePer; j++) {
+      const int tj = j * kEroBlock + threadIdx.x;
+      const int gx = x0 + tj % kTile, gy = y0 + tj / kTile;
+      const bool in = gx < W && gy < H;
+      const size_t g = in ? static_cast<size_t>(gy) * W + gx : 0;
+      th[j] = in ? h_old[g] : 0.0;
+      tc[j] = in ? cnt[g] : 0u;
+      tl[j] = in ? lvl[g] : kEscaped;
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; q++) r_s[q * kEroBlock + threadIdx.x] = rv[q];
+#pragma unroll
+    for (int j = 0; j < kTilePer; j++) {
+      hs[j * kEroBlock + threadIdx.x] = th[j];
+      c_s[j * kEroBlock + threadIdx.x] = tc[j];
+      d_s[j * kEroBlock + threadIdx.x] = tl[j];
+    }
+  }
+  __syncthreads();
+  mark_halo_chains<kEroBlock>(r_s, sh.mark);
+  __syncthreads();
+  uint32_t nhalo;
+  {
+    const uint32_t v = threadIdx.x < kRegCells / 32 ? __popc(sh.mark[threadIdx.x]) : 0u;
+    const uint32_t ex = block_excl_scan<kEroBlock>(v, &nhalo, sh.scan);
+    if (threadIdx.x < kRegCells / 32) sh.pre[threadIdx.x] = static_cast<uint16_t>(ex);
+  }
+  if (nhalo > static_cast<uint32_t>(kHaloCap)) {   // excluded by pass 1
+    if (threadIdx.x == 0) atomicOr(&hdr[2], kErrIndex);
+    return;
+  }
+  __syncthreads();
+  // elevation and level of the marked halo cells: a thread's first two are
+  // loaded together, any further ones after
+  {
+    int ia = -1, ib = -1;
+    uint32_t rest = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      const int i = q * kEroBlock + threadIdx.x;
+      if (!(sh.mark[i >> 5] & (1u << (i & 31)))) continue;
+      if (ia < 0) ia = i;
+      else if (ib < 0) ib = i;
+      else rest |= 1u << q;
+    }
+    auto gidx = [&](int i) {
+      return static_cast<size_t>(ry0 + i / kReg) * W + (rx0 + i % kReg);
+    };
+    double ha = 0.0, hb = 0.0;
+    uint32_t ca = 0, cb = 0;
+    uint8_t da = 0, db = 0;
+    if (ia >= 0) {
+      ha = h_old[gidx(ia)];
+      ca = cnt[gidx(ia)];
+      da = lvl[gidx(ia)];
+    }
+    if (ib >= 0) {
+      hb = h_old[gidx(ib)];
+      cb = cnt[gidx(ib)];
+      db = lvl[gidx(ib)];
+    }
+    if (ia >= 0) {
+      const int sl = ero_slot(sh, ia);
+      hs[sl] = ha;
+      c_s[sl] = ca;
+      d_s[sl] = da;
+      sh.hreg[sl - kTileCells] = static_cast<uint16_t>(ia);
+    }
+    if (ib >= 0) {
+      const int sl = ero_slot(sh, ib);
+      hs[sl] = hb;
+      c_s[sl] = cb;
+      d_s[sl] = db;
+      sh.hreg[sl - kTileCells] = static_cast<uint16_t>(ib);
+    }
+    while (rest) {
+      const int q = __builtin_ctz(rest);
+      rest &= rest - 1;
+      const int i = q * kEroBlock + threadIdx.x;
+      const int sl = ero_slot(sh, i);
+      hs[sl] = h_old[gidx(i)];
+      c_s[sl] = cnt[gidx(i)];
+      d_s[sl] = lvl[gidx(i)];
+      sh.hreg[sl - kTileCells] = static_cast<uint16_t>(i);
+    }
+  }
+  __syncthreads();
+  // bucket the slots by level: rank within (level, wave) by matching keys
+  // with five ballots, no atomics; then level bases
+  const int nslots = kTileCells + static_cast<int>(nhalo);
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+  int rank[kSlotPer];
+#pragma unroll
+  for (int q = 0; q < kSlotPer; q++) {
+    const int sl = q * kEroBlock + threadIdx.x;
+    const int d = sl < nslots ? d_s[sl] : 0xFF;
+    const bool valid = d < kKeys;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 5; b++) {
+      const uint64_t bb = __ballot((d >> b) & 1);
+      m &= ((d >> b) & 1) ? bb : ~bb;
+    }
+    const int before = __popcll(m & lt);
+    const int base = valid ? wcount[d][wave] : 0;
+    rank[q] = valid ? base + before : -1;
+    if (valid && before == 0) wcount[d][wave] = base + __popcll(m);
   }
   __syncthreads();
   if (threadIdx.x < kKeys) {
@@ -884,10 +1062,11 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
   }
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < kPer; q++) {
-    const int i = q * kEroBlock + threadIdx.x;
-    const int d = d_s[i];
-    if (rank[q] >= 0) list[loff[d] + wcount[d][wave] + rank[q]] = static_cast<uint16_t>(i);
+  for (int q = 0; q < kSlotPer; q++) {
+    const int sl = q * kEroBlock + threadIdx.x;
+    if (rank[q] < 0) continue;
+    const int d = d_s[sl];
+    list[loff[d] + wcount[d][wave] + rank[q]] = static_cast<uint16_t>(sl);
   }
   if (threadIdx.x < kKeys)
     for (int e = loff[threadIdx.x] + ltotal[threadIdx.x]; e < loff[threadIdx.x + 1]; e++)
@@ -895,17 +1074,20 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
   __syncthreads();
   const int nchunks = loff[kKeys] / kWave, root_chunks = loff[1] / kWave;
   for (int ch = wave; ch < nchunks; ch += kWaves) {
-    const int i = list[ch * kWave + lane];
-    if (i == 0xFFFF) continue;
+    const int sl = list[ch * kWave + lane];
+    if (sl == 0xFFFF) continue;
+    const int i = sl < kTileCells
+                      ? (sl / kTile + kHalo) * kReg + sl % kTile + kHalo
+                      : sh.hreg[sl - kTileCells];
+    int lx = i % kReg, ly = i / kReg;
+    const int gx = rx0 + lx, gy = ry0 + ly;
     if (ch < root_chunks) {
       // level 0: uplift of interior outlets; rings 0 and 1 stay fixed
-      const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
-      if (gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2) hs[i] += k.ud;
+      if (gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2) hs[sl] += k.ud;
     } else {
       const int r = r_s[i];
-      int lx = i % kReg, ly = i / kReg;
-      if (region_step(r, lx, ly)) {          // else the receiver is not needed here
-        const int nb = ly * kReg + lx;
+      if (region_step(r, lx, ly)) {          // always, for a cell on a tile chain
+        const int nb = ero_slot(sh, ly * kReg + lx);
         int spins = 0;
         while (__hip_atomic_load(&d_s[nb], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) !=
                kDone) {
@@ -915,21 +1097,21 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
           }
           __builtin_amdgcn_s_sleep(1);
         }
-        const uint32_t a = c_s[i];
+        const uint32_t a = c_s[sl];
         const int ti = 2 * static_cast<int>(a) + (r & 1);
-        const double fact = a < kFactCache ? ft_s[ti] : ftab[ti];
-        hs[i] = newton_update<NMODE>(hs[i] + k.ud, hs[nb], fact, k, hdr);
+        const double fact = a < kEroFactCache ? ft_s[ti] : ftab[ti];
+        hs[sl] = newton_update<NMODE>(hs[sl] + k.ud, hs[nb], fact, k, hdr);
       }
     }
-    __hip_atomic_store(&d_s[i], kDone, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_store(&d_s[sl], kDone, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   __syncthreads();
-  for (int it = 0; it < kTile * kTile / kEroBlock; it++) {
-    const int i = it * kEroBlock + threadIdx.x;
-    const int tx = i % kTile, ty = i / kTile;
-    const int gx = x0 + tx, gy = y0 + ty;
+#pragma unroll
+  for (int j = 0; j < kTilePer; j++) {
+    const int tj = j * kEroBlock + threadIdx.x;
+    const int gx = x0 + tj % kTile, gy = y0 + tj / kTile;
     if (gx >= W || gy >= H) continue;
-    h_new[static_cast<size_t>(gy) * W + gx] = hs[(ty + kHalo) * kReg + tx + kHalo];
+    h_new[static_cast<size_t>(gy) * W + gx] = hs[tj];
   }
 }
 
