@@ -639,7 +639,7 @@ constexpr int kTile = 64;
 constexpr int kHalo = 16;
 constexpr int kReg = kTile + 2 * kHalo;       // region side
 constexpr int kRegCells = kReg * kReg;
-constexpr int kAccBlock = 512;
+constexpr int kAccBlock = 1024;
 constexpr int kEroBlock = 1024;
 constexpr uint8_t kOutside = 0xFD;            // region cell beyond the grid
 constexpr uint8_t kEscaped = 0xFF;            // chain leaves the region
