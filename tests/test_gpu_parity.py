@@ -315,3 +315,24 @@ def test_benchmark_grid_takes_tile_path(oracle):
         m.step(1)
         compare_step(m, st, "2000 tile step")
         np.testing.assert_array_equal(m.getH().ravel(), st["h"])
+
+
+@pytest.mark.parametrize("W,H", [(300, 260), (130, 70)])
+def test_receivers_of_consecutive_tile_steps(oracle, W, H):
+    """Consecutive tile-path steps without set_h (the tile path alternates two
+    elevation buffers): each step's receivers are the reference's receivers
+    of that step's input, also after the device pointer was handed out."""
+    note("consecutive-step receivers %dx%d" % (W, H))
+    op = oracle_params(oracle, 1)
+    with fa.FastScape(W, H, gpu_params(1), engine_seed=3) as m:
+        m.step(2)
+        for k in range(4):
+            h_in = m.getH().ravel().copy()
+            m.step(1)
+            np.testing.assert_array_equal(m.receivers(), oracle.receivers(h_in, W, H, op),
+                                          err_msg="step %d" % k)
+        assert m.path_stats()["tile"] >= 5
+        m.device_h()
+        h_in = m.getH().ravel().copy()
+        m.step(1)
+        np.testing.assert_array_equal(m.receivers(), oracle.receivers(h_in, W, H, op))
