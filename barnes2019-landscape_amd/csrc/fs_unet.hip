@@ -112,6 +112,122 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ in
 }
 
 // ---------------------------------------------------------------------------
+// The same convolution with operands staged in LDS: a workgroup computes an
+// 8 x 16 pixel tile (one image) x NB outputs.  For each 32-channel step the
+// 10 x 18 input patch (tile + one-pixel halo, zero padded) and the 9 taps'
+// weights go to LDS once and every tap's MFMA operands are read from there,
+// so each input value is fetched once per tile instead of once per tap and
+// wave.  Wave w owns tile rows 2w and 2w+1 (two 16-pixel MFMA rows) and all
+// NB outputs.  Rows are padded to 40 bf16 (80 bytes) against bank conflicts.
+// Needs nout % NB == 0 and cin % 8 == 0.
+// ---------------------------------------------------------------------------
+constexpr int kTPH = 8, kTPW = 16;               // pixel tile
+constexpr int kPatchH = kTPH + 2, kPatchW = kTPW + 2;
+constexpr int kKC = 32;                           // input channels per step
+constexpr int kCS = kKC + 8;                      // padded channel stride in LDS
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_conv3x3_t(const uint16_t *__restrict__ in, int cin,
+                                                   const uint16_t *__restrict__ wt,
+                                                   const float *__restrict__ bias,
+                                                   uint16_t *__restrict__ out, int nout, int H,
+                                                   int W, int relu, int tiles_y) {
+  constexpr int NT = NB / 16;
+  __shared__ __attribute__((aligned(16))) uint16_t patch[kPatchH * kPatchW * kCS];
+  __shared__ __attribute__((aligned(16))) uint16_t ws[9 * NB * kCS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int x0 = blockIdx.x * kTPW;
+  const int ty = blockIdx.y % tiles_y;
+  const long b = blockIdx.y / tiles_y;
+  const int y0 = ty * kTPH;
+  const int n0 = blockIdx.z * NB;
+  f32x4 acc[2][NT];
+#pragma unroll
+  for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // the next step's patch and weights are loaded into registers while the
+  // current step is computed from LDS
+  constexpr int kPV = (kPatchH * kPatchW * 4 + 255) / 256;
+  constexpr int kWV = (9 * NB * 4 + 255) / 256;
+  uint4 pv[kPV], wv[kWV];
+  auto load = [&](int c0) {
+#pragma unroll
+    for (int k = 0; k < kPV; k++) {
+      const int v = k * 256 + threadIdx.x;
+      const int pix = v >> 2, c8 = v & 3;
+      const int py = pix / kPatchW, px = pix % kPatchW;
+      const int yy = y0 - 1 + py, xx = x0 - 1 + px;
+      const int c = c0 + 8 * c8;
+      pv[k] = make_uint4(0u, 0u, 0u, 0u);
+      if (v < kPatchH * kPatchW * 4 && yy >= 0 && yy < H && xx >= 0 && xx < W && c < cin)
+        pv[k] = *reinterpret_cast<const uint4 *>(in + ((b * H + yy) * W + xx) * cin + c);
+    }
+#pragma unroll
+    for (int k = 0; k < kWV; k++) {
+      const int v = k * 256 + threadIdx.x;
+      const int row = v >> 2, c8 = v & 3;           // row = tap * NB + output
+      const int t = row / NB, o = row % NB;
+      const int c = c0 + 8 * c8;
+      wv[k] = make_uint4(0u, 0u, 0u, 0u);
+      if (v < 9 * NB * 4 && c < cin)
+        wv[k] = *reinterpret_cast<const uint4 *>(wt + (static_cast<long>(t) * nout + n0 + o) * cin + c);
+    }
+  };
+  load(0);
+  for (int c0 = 0; c0 < cin; c0 += kKC) {
+#pragma unroll
+    for (int k = 0; k < kPV; k++) {
+      const int v = k * 256 + threadIdx.x;
+      if (v < kPatchH * kPatchW * 4)
+        *reinterpret_cast<uint4 *>(&patch[(v >> 2) * kCS + 8 * (v & 3)]) = pv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kWV; k++) {
+      const int v = k * 256 + threadIdx.x;
+      if (v < 9 * NB * 4) *reinterpret_cast<uint4 *>(&ws[(v >> 2) * kCS + 8 * (v & 3)]) = wv[k];
+    }
+    __syncthreads();
+    if (c0 + kKC < cin) load(c0 + kKC);
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+      const int dy = t / 3, dx = t % 3;
+      uint4 av[2];
+#pragma unroll
+      for (int mt = 0; mt < 2; mt++) {
+        const int pr = 2 * wave + mt;
+        av[mt] = *reinterpret_cast<const uint4 *>(&patch[((pr + dy) * kPatchW + r + dx) * kCS + 8 * g]);
+      }
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++) {
+        const uint4 bv = *reinterpret_cast<const uint4 *>(&ws[(t * NB + 16 * nt + r) * kCS + 8 * g]);
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) acc[mt][nt] = mfma16(av[mt], bv, acc[mt][nt]);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int mt = 0; mt < 2; mt++) {
+    const int yy = y0 + 2 * wave + mt;
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+      const int n = n0 + 16 * nt + r;
+      const float bb = bias ? bias[n] : 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int xx = x0 + 4 * g + i;
+        if (yy >= H || xx >= W) continue;
+        float v = acc[mt][nt][i] + bb;
+        if (relu) v = fmaxf(v, 0.f);
+        out[((b * H + yy) * W + xx) * nout + n] = f2bf(v);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient of a 3x3 convolution, split over pixel chunks:
 //   part[chunk][t][co][ci] = sum_{p in chunk} dz[p][co] * x[p + off_t][ci]
 // GEMM with M = co, N = ci (64 x 64 per workgroup, 32 x 32 per wave),
@@ -559,18 +675,33 @@ long pixels(const fs_unet *u, int level, int B) {
   return static_cast<long>(B) * (u->H >> level) * (u->W >> level);
 }
 
-int conv_fwd(fs_unet *u, const Conv &c, const Tensor &in, Tensor &out, int B) {
-  const int Hl = u->H >> c.level, Wl = u->W >> c.level;
-  const long M = pixels(u, c.level, B);
-  const float *bias = u->w + c.boff;
-  if (c.cout % 64 == 0) {
-    hipLaunchKernelGGL(k_conv3x3<4>, dim3((M + 63) / 64, c.cout / 64), dim3(256), 0, u->stream,
-                       in.d, c.cin, c.wf, bias, out.d, c.cout, Hl, Wl, 1, M);
+// out = conv3x3(in) with weights wt [tap][nout][cin]: the LDS-tiled kernel
+// for images of at least one 8 x 16 tile, the direct one for the small
+// deep levels.
+int launch_conv(fs_unet *u, const uint16_t *in, int cin, const uint16_t *wt, const float *bias,
+                uint16_t *out, int nout, int Hl, int Wl, int relu, int B) {
+  const long M = static_cast<long>(B) * Hl * Wl;
+  if (Hl >= kTPH && Wl >= kTPW) {
+    const int tx = (Wl + kTPW - 1) / kTPW, ty = (Hl + kTPH - 1) / kTPH;
+    if (nout % 64 == 0)
+      hipLaunchKernelGGL(k_conv3x3_t<64>, dim3(tx, ty * B, nout / 64), dim3(256), 0, u->stream,
+                         in, cin, wt, bias, out, nout, Hl, Wl, relu, ty);
+    else
+      hipLaunchKernelGGL(k_conv3x3_t<32>, dim3(tx, ty * B, nout / 32), dim3(256), 0, u->stream,
+                         in, cin, wt, bias, out, nout, Hl, Wl, relu, ty);
+  } else if (nout % 64 == 0) {
+    hipLaunchKernelGGL(k_conv3x3<4>, dim3((M + 63) / 64, nout / 64), dim3(256), 0, u->stream, in,
+                       cin, wt, bias, out, nout, Hl, Wl, relu, M);
   } else {
-    hipLaunchKernelGGL(k_conv3x3<2>, dim3((M + 63) / 64, c.cout / 32), dim3(256), 0, u->stream,
-                       in.d, c.cin, c.wf, bias, out.d, c.cout, Hl, Wl, 1, M);
+    hipLaunchKernelGGL(k_conv3x3<2>, dim3((M + 63) / 64, nout / 32), dim3(256), 0, u->stream, in,
+                       cin, wt, bias, out, nout, Hl, Wl, relu, M);
   }
   return check_launch_u();
+}
+
+int conv_fwd(fs_unet *u, const Conv &c, const Tensor &in, Tensor &out, int B) {
+  const int Hl = u->H >> c.level, Wl = u->W >> c.level;
+  return launch_conv(u, in.d, c.cin, c.wf, u->w + c.boff, out.d, c.cout, Hl, Wl, 1, B);
 }
 
 // Gradients of one conv+ReLU layer from gout (gradient of its output):
@@ -600,16 +731,7 @@ int conv_bwd(fs_unet *u, const Conv &c, const Tensor &in, const Tensor &out, con
   hipLaunchKernelGGL(k_bias_grad, dim3((M + bchunk - 1) / bchunk), dim3(256), 0, u->stream,
                      u->gz.d, c.cout, u->g + c.boff, M, bchunk);
   FS_TRY(check_launch_u());
-  if (gin) {
-    if (c.cin % 64 == 0) {
-      hipLaunchKernelGGL(k_conv3x3<4>, dim3((M + 63) / 64, c.cin / 64), dim3(256), 0, u->stream,
-                         u->gz.d, c.cout, c.wb, nullptr, gin->d, c.cin, Hl, Wl, 0, M);
-    } else {
-      hipLaunchKernelGGL(k_conv3x3<2>, dim3((M + 63) / 64, c.cin / 32), dim3(256), 0, u->stream,
-                         u->gz.d, c.cout, c.wb, nullptr, gin->d, c.cin, Hl, Wl, 0, M);
-    }
-    FS_TRY(check_launch_u());
-  }
+  if (gin) FS_TRY(launch_conv(u, u->gz.d, c.cout, c.wb, nullptr, gin->d, c.cin, Hl, Wl, 0, B));
   return FS_OK;
 }
 
