@@ -1,7 +1,8 @@
 // fs_receiver.h -- the D8 receiver choice of ComputeReceivers
-// (fastscape_RB.cpp:160-182) with two IEEE divisions per cell instead of
-// eight.  Compiled into the device kernels and, through hipcc's host pass,
-// into the CPU equivalence test (tests/test_receiver_rule.py).
+// (fastscape_RB.cpp:160-182), usually without any division and never with
+// more than the two-division fallback plus near-tie checks.  Compiled into
+// the device kernels and, through hipcc's host pass, into the CPU
+// equivalence test (tests/test_receiver_rule.py).
 #ifndef FS_RECEIVER_H
 #define FS_RECEIVER_H
 
@@ -57,7 +58,11 @@ __host__ __device__ inline int group_best(const double d[8], int first, double d
   return kNoReceiver;   // unreachable: the maximum itself matches
 }
 
-__host__ __device__ inline int receiver_fast(double hc, const double nb[8], double dc, double dd) {
+// Two IEEE divisions per cell (one per distance group) plus the near-tie
+// checks of group_best; correct for every input, including NaN, infinities,
+// subnormal and overflowing quotients.
+__host__ __device__ inline int receiver_two_div(double hc, const double nb[8], double dc,
+                                                double dd) {
   double d[8];
   for (int n = 0; n < 8; n++) d[n] = hc - nb[n];
   double qc, qd;
@@ -67,6 +72,54 @@ __host__ __device__ inline int receiver_fast(double hc, const double nb[8], doub
   if (nd == kNoReceiver || qc > qd) return nc;
   if (nc == kNoReceiver || qd > qc) return nd;
   return nc < nd ? nc : nd;
+}
+
+// The group's best direction when it can be told without dividing: the
+// first drop within 2^-50*A of the group's largest drop A is the best
+// direction if it is A itself (earlier drops are too far below A to round to
+// the same quotient).  -1 when that first drop is a smaller near-tie, which
+// only a division can settle.
+__host__ __device__ inline int group_first(const double d[8], int first, double A) {
+  const double window = A - A * 0x1p-50;
+  for (int n = first; n < 8; n += 2)
+    if (d[n] >= window) return d[n] == A ? n : -1;
+  return -1;   // unreachable: A itself is in the window
+}
+
+// The reference's direction, decided without division whenever every
+// quotient involved is a normal number and the two groups' slopes are not
+// near-tied.  For drops Ac, Ad and divisors dc, dd in range, P = RN(Ac*dd)
+// and Q = RN(Ad*dc) carry relative errors of at most 2^-53 each, so
+// P > RN(Q*(1 + 2^-48)) gives Ac/dc > (Ad/dd)*(1 + 2^-49), hence
+// RN(Ac/dc) > RN(Ad/dd) strictly: the cardinal group's quotient wins without
+// a tie.  Anything closer (or out of range) goes to receiver_two_div.
+__host__ __device__ inline int receiver_fast(double hc, const double nb[8], double dc, double dd) {
+  double d[8];
+  for (int n = 0; n < 8; n++) d[n] = hc - nb[n];
+  double Ac = -__builtin_huge_val(), Ad = -__builtin_huge_val();
+  for (int n = 0; n < 8; n += 2)
+    if (d[n] > Ac) Ac = d[n];
+  for (int n = 1; n < 8; n += 2)
+    if (d[n] > Ad) Ad = d[n];
+  const bool pc = Ac > 0.0, pd = Ad > 0.0;
+  if (!pc && !pd) return kNoReceiver;
+  // quotients of drops in [2^-900, 2^900] by divisors in [2^-60, 2^60] are
+  // normal and nonzero: a positive drop is a positive slope
+  const bool okc = !pc || (Ac >= 0x1p-900 && Ac <= 0x1p900);
+  const bool okd = !pd || (Ad >= 0x1p-900 && Ad <= 0x1p900);
+  const bool okr = dc >= 0x1p-60 && dc <= 0x1p60 && dd >= 0x1p-60 && dd <= 0x1p60;
+  if (okc && okd && okr) {
+    const int bc = pc ? group_first(d, 0, Ac) : kNoReceiver;
+    const int bd = pd ? group_first(d, 1, Ad) : kNoReceiver;
+    if (bc >= 0 && bd >= 0) {
+      if (!pd) return bc;
+      if (!pc) return bd;
+      const double P = Ac * dd, Q = Ad * dc;
+      if (P > Q * (1.0 + 0x1p-48)) return bc;
+      if (Q > P * (1.0 + 0x1p-48)) return bd;
+    }
+  }
+  return receiver_two_div(hc, nb, dc, dd);
 }
 
 }  // namespace fs

@@ -1,4 +1,5 @@
-// CPU check of fs::receiver_fast against fs::receiver_reference (the
+// CPU check of fs::receiver_fast (division-free decision) and
+// fs::receiver_two_div (its fallback) against fs::receiver_reference (the
 // reference's eight-division loop, fastscape_RB.cpp:160-182) on random,
 // tied, near-tied and special inputs.  Built by tests/test_receiver_rule.py
 // with hipcc's host pass.  Prints "cases mismatches".
@@ -28,14 +29,15 @@ static inline double step_ulps(double x, int k) {
 int main(int argc, char **argv) {
   const long n_cases = argc > 1 ? std::atol(argv[1]) : 20000000;
   const double SQRT2 = 1.414213562373095048801688724209698078569671875376948;
-  const double dcs[4] = {1.0, 100.0, 30.0, 0.001};
+  // the last two lie outside the division-free decision's divisor range
+  const double dcs[6] = {1.0, 100.0, 30.0, 0.001, 1e-30, 1e30};
   const double specials[10] = {0.0, -0.0, std::numeric_limits<double>::infinity(),
                                -std::numeric_limits<double>::infinity(),
                                std::numeric_limits<double>::quiet_NaN(), 4.9e-324, 1e-310,
                                1e308, -1e308, 1.0};
   long bad = 0;
   for (long t = 0; t < n_cases; t++) {
-    const double dc = dcs[t & 3], dd = dc * SQRT2;
+    const double dc = dcs[t % 6], dd = dc * SQRT2;
     double hc, nb[8];
     const int kind = static_cast<int>(rnd() % 6);
     if (kind == 0) {                         // uniform
@@ -65,9 +67,11 @@ int main(int argc, char **argv) {
     }
     const int a = fs::receiver_reference(hc, nb, dc, dd);
     const int b = fs::receiver_fast(hc, nb, dc, dd);
-    if (a != b) {
+    const int c = fs::receiver_two_div(hc, nb, dc, dd);
+    if (a != b || a != c) {
       if (bad < 10) {
-        std::printf("mismatch kind %d dc %g: ref %d fast %d hc %a nb", kind, dc, a, b, hc);
+        std::printf("mismatch kind %d dc %g: ref %d fast %d two-div %d hc %a nb", kind, dc, a, b, c,
+                    hc);
         for (double v : nb) std::printf(" %a", v);
         std::printf("\n");
       }
