@@ -643,6 +643,7 @@ constexpr int kAccBlock = 1024;
 constexpr int kEroBlock = 1024;
 constexpr uint8_t kOutside = 0xFD;            // region cell beyond the grid
 constexpr uint8_t kEscaped = 0xFF;            // chain leaves the region
+constexpr uint8_t kLeaves = 0xFE;             // receiver lies outside the region
 constexpr int kFactCache = 256;               // erosion factors cached in LDS
 
 // Region-local neighbour step in direction r; false if it leaves the region.
@@ -664,42 +665,46 @@ __device__ __forceinline__ int xcd_tile(int b, int ntiles) {
   return x * q + min(x, rem) + k;
 }
 
-// Load the receivers (and optionally elevations and levels) of the region:
-// every load of a thread is issued before the first LDS store.
-template <int BLOCK, bool WITH_H>
-__device__ __forceinline__ void load_region(const uint8_t *__restrict__ rcv,
-                                            const double *__restrict__ h,
-                                            const uint8_t *__restrict__ lvl,
-                                            const uint32_t *__restrict__ cnt, uint8_t *r_s,
-                                            double *hs, uint8_t *d_s, uint32_t *c_s, int rx0,
-                                            int ry0, int W, int H) {
+// Region-local index offset of direction r (0..7), from one packed constant.
+__device__ __forceinline__ int region_offset(int r) {
+  constexpr int8_t kOff[8] = {-1, -kReg - 1, -kReg, -kReg + 1, 1, kReg + 1, kReg, kReg - 1};
+  constexpr uint64_t kPacked =
+      static_cast<uint64_t>(static_cast<uint8_t>(kOff[0])) |
+      static_cast<uint64_t>(static_cast<uint8_t>(kOff[1])) << 8 |
+      static_cast<uint64_t>(static_cast<uint8_t>(kOff[2])) << 16 |
+      static_cast<uint64_t>(static_cast<uint8_t>(kOff[3])) << 24 |
+      static_cast<uint64_t>(static_cast<uint8_t>(kOff[4])) << 32 |
+      static_cast<uint64_t>(static_cast<uint8_t>(kOff[5])) << 40 |
+      static_cast<uint64_t>(static_cast<uint8_t>(kOff[6])) << 48 |
+      static_cast<uint64_t>(static_cast<uint8_t>(kOff[7])) << 56;
+  return static_cast<int8_t>(static_cast<uint8_t>(kPacked >> (8 * r)));
+}
+
+// Load the receivers of the region into r_s (every load of a thread is
+// issued before the first LDS store).  A receiver outside the region is
+// stored as kLeaves, so walks over r_s follow region_offset without bounds
+// checks: they stop at any code >= 8 (kNoFlow: an outlet; kLeaves: the chain
+// leaves the region; kOutside: beyond the grid, never reached by a walk).
+template <int BLOCK>
+__device__ __forceinline__ void load_region_receivers(const uint8_t *__restrict__ rcv,
+                                                      uint8_t *r_s, int rx0, int ry0, int W,
+                                                      int H) {
   constexpr int kPer = kRegCells / BLOCK;
   static_assert(kRegCells % BLOCK == 0, "region must split evenly over the block");
-  uint8_t rv[kPer], dv[WITH_H ? kPer : 1];
-  double hv[WITH_H ? kPer : 1];
-  uint32_t cv[WITH_H ? kPer : 1];
+  uint8_t rv[kPer];
 #pragma unroll
   for (int k = 0; k < kPer; k++) {
     const int i = k * BLOCK + threadIdx.x;
     const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
     const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
-    const size_t g = in ? static_cast<size_t>(gy) * W + gx : 0;
-    rv[k] = in ? rcv[g] : kOutside;
-    if (WITH_H) {
-      hv[k] = in ? h[g] : 0.0;
-      dv[k] = in ? lvl[g] : kEscaped;
-      cv[k] = in ? cnt[g] : 0u;
-    }
+    rv[k] = in ? rcv[static_cast<size_t>(gy) * W + gx] : kOutside;
   }
 #pragma unroll
   for (int k = 0; k < kPer; k++) {
     const int i = k * BLOCK + threadIdx.x;
-    r_s[i] = rv[k];
-    if (WITH_H) {
-      hs[i] = hv[k];
-      d_s[i] = dv[k];
-      c_s[i] = cv[k];
-    }
+    int lx = i % kReg, ly = i / kReg;
+    const int r = rv[k];
+    r_s[i] = (r < 8 && !region_step(r, lx, ly)) ? kLeaves : rv[k];
   }
 }
 
@@ -763,39 +768,37 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
   if (threadIdx.x < kHalo + 1) sh.hist[threadIdx.x] = 0;
   if (threadIdx.x < kRegCells / 32) sh.mark[threadIdx.x] = 0;
   for (int it = 0; it < kRegCells / BLOCK; it++) c_s[it * BLOCK + threadIdx.x] = 0;
-  load_region<BLOCK, false>(rcv, nullptr, nullptr, nullptr, r_s, nullptr, nullptr, nullptr,
-                            rx0, ry0, W, H);
+  load_region_receivers<BLOCK>(rcv, r_s, rx0, ry0, W, H);
   __syncthreads();
   // kWalkers chains per thread advance together so that their dependent LDS
-  // reads overlap; a walker stops at an outlet, at the region edge, or
-  // after kHalo hops.
+  // reads overlap; a walker stops at an outlet, where its chain leaves the
+  // region, or after kHalo hops.
   constexpr int kPer = kRegCells / BLOCK;
   constexpr int kWalkers = BLOCK == 512 ? 6 : kPer;
   static_assert(kPer % kWalkers == 0, "walkers must split the thread's cells evenly");
   int over = 0;
   for (int g = 0; g < kPer; g += kWalkers) {
-    int wx[kWalkers], wy[kWalkers], hops[kWalkers];
+    int x[kWalkers], hops[kWalkers];
     uint32_t live = 0, rooted = 0;
 #pragma unroll
     for (int j = 0; j < kWalkers; j++) {
-      const int i = (g + j) * BLOCK + threadIdx.x;
-      wx[j] = i % kReg;
-      wy[j] = i / kReg;
+      x[j] = (g + j) * BLOCK + threadIdx.x;
       hops[j] = 0;
-      if (r_s[i] != kOutside) live |= 1u << j;
+      if (r_s[x[j]] != kOutside) live |= 1u << j;
     }
     while (live) {
 #pragma unroll
       for (int j = 0; j < kWalkers; j++) {
         if (!(live & (1u << j))) continue;
-        const int x = wy[j] * kReg + wx[j];
-        atomicAdd(&c_s[x], 1u);
-        const int r = r_s[x];
+        atomicAdd(&c_s[x[j]], 1u);
+        const int r = r_s[x[j]];
         if (r >= 8) {
-          rooted |= 1u << j;
+          if (r == kNoFlow) rooted |= 1u << j;
           live &= ~(1u << j);
-        } else if (!region_step(r, wx[j], wy[j]) || ++hops[j] > kHalo) {
+        } else if (++hops[j] > kHalo) {
           live &= ~(1u << j);
+        } else {
+          x[j] += region_offset(r);
         }
       }
     }
