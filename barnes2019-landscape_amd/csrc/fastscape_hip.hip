@@ -72,6 +72,7 @@ struct fs_model {
   int32_t *list = nullptr, *stack = nullptr, *hdr = nullptr, *levels_d = nullptr;
   uint32_t *cnt = nullptr;
   double *ftab = nullptr;   // erosion factor per (upstream count, diagonal)
+  TilePlan *plan = nullptr; // per-tile erosion plan of the tile path
   uint64_t *status = nullptr;
   unsigned long long *ticket = nullptr;
 
@@ -183,7 +184,7 @@ void free_model(fs_model *m) {
                     static_cast<void *>(m->stack), static_cast<void *>(m->hdr),
                     static_cast<void *>(m->levels_d), static_cast<void *>(m->cnt),
                     static_cast<void *>(m->ftab), static_cast<void *>(m->status),
-                    static_cast<void *>(m->ticket)})
+                    static_cast<void *>(m->ticket), static_cast<void *>(m->plan)})
     if (ptr) (void)hipFree(ptr);
   if (m->hdr_h) (void)hipHostFree(m->hdr_h);
   if (m->levels_h) (void)hipHostFree(m->levels_h);
@@ -248,6 +249,9 @@ int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_
   FS_TRY(fs::dev_alloc(&m->levels_d, FS_MAX_LEVELS + 4));
   FS_TRY(fs::dev_alloc(&m->status, ntiles));
   FS_TRY(fs::dev_alloc(&m->ticket, 1));
+  const size_t plan_tiles = static_cast<size_t>((m->W + kTile - 1) / kTile) *
+                            static_cast<size_t>((m->H + kTile - 1) / kTile);
+  FS_TRY(fs::dev_alloc(&m->plan, plan_tiles));
   if (hipHostMalloc(reinterpret_cast<void **>(&m->hdr_h),
                     (kHdrInts + FS_MAX_LEVELS + 3) * sizeof(int32_t)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void **>(&m->levels_h),
@@ -259,6 +263,7 @@ int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_
   FS_HIP_TRY(hipMemsetAsync(m->hdr, 0, (kHdrInts + FS_MAX_LEVELS + 3) * sizeof(int32_t), m->stream));
   FS_HIP_TRY(hipMemsetAsync(m->status, 0, ntiles * sizeof(uint64_t), m->stream));
   FS_HIP_TRY(hipMemsetAsync(m->ticket, 0, sizeof(unsigned long long), m->stream));
+  FS_HIP_TRY(hipMemsetAsync(m->plan, 0, plan_tiles * sizeof(TilePlan), m->stream));
   FS_HIP_TRY(hipMemsetAsync(m->cnt, 0, N * sizeof(uint32_t), m->stream));
   {
     std::vector<double> tab;
@@ -514,7 +519,7 @@ int level_path(fs_model *m, hipEvent_t *ev) {
 template <int NMODE>
 int launch_tile_erode(fs_model *m, int tiles, int tiles_x) {
   hipLaunchKernelGGL(k_tile_erode<NMODE>, dim3(tiles), dim3(kEroBlock), 0, m->stream, m->h, m->h_alt,
-                     m->rcv, m->lvl, m->cnt, m->ftab, m->W, m->H, tiles_x, tiles, m->ek, m->hdr);
+                     m->plan, m->lvl, m->cnt, m->ftab, m->W, m->H, tiles_x, tiles, m->ek, m->hdr);
   return check_launch();
 }
 
@@ -529,8 +534,8 @@ int tile_path(fs_model *m, hipEvent_t *ev, bool *done) {
   hipLaunchKernelGGL(k_fill_counts, dim3(1), dim3(64), 0, s, m->hdr, 0);
   FS_TRY(check_launch());
   FS_TRY(kt_begin(m, FS_K_TILE_ACC));
-  hipLaunchKernelGGL(k_tile_acc, dim3(tiles), dim3(kAccBlock), 0, s, m->rcv, m->cnt, m->lvl, m->hdr,
-                     m->W, m->H, tiles_x, tiles);
+  hipLaunchKernelGGL(k_tile_acc, dim3(tiles), dim3(kAccBlock), 0, s, m->rcv, m->cnt, m->lvl, m->plan,
+                     m->hdr, m->W, m->H, tiles_x, tiles);
   FS_TRY(check_launch());
   FS_TRY(kt_end(m));
   FS_HIP_TRY(hipEventRecord(ev[2], s));

@@ -744,131 +744,204 @@ __device__ __forceinline__ void mark_halo_chains(const uint8_t *r_s, uint32_t *m
   }
 }
 
+// Per-tile erosion plan, written by the accumulation pass for the erosion
+// pass.  Slots: the tile's cells at their tile index, then the marked halo
+// cells in region order.  rslot holds each slot's receiver slot with
+// kPlanDiag set for a diagonal receiver, or kPlanRoot for an outlet.
+constexpr int kTileCells = kTile * kTile;
+constexpr int kSlots = kTileCells + kHaloCap;
+constexpr uint16_t kPlanDiag = 0x8000u;
+constexpr uint16_t kPlanRoot = 0x7FFFu;
+constexpr uint16_t kPlanSlot = 0x7FFFu;       // slot bits of an rslot entry
+static_assert(kSlots < kPlanRoot, "slot numbers must fit below kPlanRoot");
+struct alignas(16) TilePlan {
+  uint16_t rslot[kSlots];
+  uint16_t hreg[kHaloCap];     // region index of each halo slot
+  uint8_t hlvl[kHaloCap];      // level of each halo slot
+  int32_t nhalo;
+};
+
 struct AccShared {
   uint8_t r_s[kRegCells];
   uint32_t c_s[kRegCells];
   uint32_t mark[kRegCells / 32];
-  int over, nmarked;
+  uint16_t pre[kRegCells / 32];   // marks before each mark word
+  uint32_t scan[1];               // number of marks
+  int over;
   int hist[kHalo + 1];
 };
 
+// Slot of region cell c in the plan: a tile cell, or a marked halo cell.
+__device__ __forceinline__ int plan_slot(const uint32_t *mark, const uint16_t *pre, int c) {
+  const int lx = c % kReg, ly = c / kReg;
+  if (in_tile(lx, ly)) return (ly - kHalo) * kTile + (lx - kHalo);
+  return kTileCells + pre[c >> 5] + __popc(mark[c >> 5] & ((1u << (c & 31)) - 1u));
+}
+
+// rslot entry of region cell c (a tile cell or a marked halo cell).
+__device__ __forceinline__ uint16_t plan_receiver(const uint8_t *r_s, const uint32_t *mark,
+                                                  const uint16_t *pre, int c) {
+  const int r = r_s[c];
+  if (r >= 8) return kPlanRoot;   // an outlet (a receiver beyond the region makes the tile over)
+  const int s = plan_slot(mark, pre, c + region_offset(r));
+  return static_cast<uint16_t>(s | ((r & 1) ? kPlanDiag : 0u));
+}
+
 // Returns nonzero (block-uniform) when a tile cell is deeper than kHalo or
-// the tile's chains need more than kHaloCap halo cells in the erosion pass.
-// With level_hist, also adds the tile cells' level histogram to it.
+// the tile's chains need more than kHaloCap halo cells in the erosion pass;
+// otherwise also writes the tile's erosion plan.  With level_hist, adds the
+// tile cells' level histogram to it.
 template <int BLOCK>
 __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__restrict__ rcv,
                                              uint32_t *__restrict__ cnt,
-                                             uint8_t *__restrict__ lvl, int W, int H, int t,
+                                             uint8_t *__restrict__ lvl,
+                                             TilePlan *__restrict__ plan, int W, int H, int t,
                                              int tiles_x, int32_t *level_hist) {
+  static_assert(BLOCK == kAccBlock, "shared layout is sized for kAccBlock");
   uint8_t *r_s = sh.r_s;
   uint32_t *c_s = sh.c_s;
   const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
   const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
-  if (threadIdx.x == 0) sh.over = sh.nmarked = 0;
+  if (threadIdx.x == 0) sh.over = 0;
   if (threadIdx.x < kHalo + 1) sh.hist[threadIdx.x] = 0;
   if (threadIdx.x < kRegCells / 32) sh.mark[threadIdx.x] = 0;
   for (int it = 0; it < kRegCells / BLOCK; it++) c_s[it * BLOCK + threadIdx.x] = 0;
   load_region_receivers<BLOCK>(rcv, r_s, rx0, ry0, W, H);
   __syncthreads();
-  // kWalkers chains per thread advance together so that their dependent LDS
-  // reads overlap; a walker stops at an outlet, where its chain leaves the
-  // region, or after kHalo hops.
+  // Every region cell walks its receiver chain and adds 1 to each cell it
+  // passes, itself included; the thread's chains advance together so that
+  // their dependent LDS reads overlap.  A walker stops at an outlet, where
+  // its chain leaves the region, or after kHalo hops.
   constexpr int kPer = kRegCells / BLOCK;
-  constexpr int kWalkers = BLOCK == 512 ? 6 : kPer;
-  static_assert(kPer % kWalkers == 0, "walkers must split the thread's cells evenly");
-  int over = 0;
-  for (int g = 0; g < kPer; g += kWalkers) {
-    int x[kWalkers], hops[kWalkers];
-    uint32_t live = 0, rooted = 0;
+  static_assert(kPer <= 16, "walkers per thread");
+  int x[kPer], hops[kPer];
+  uint32_t live = 0, rooted = 0;
 #pragma unroll
-    for (int j = 0; j < kWalkers; j++) {
-      x[j] = (g + j) * BLOCK + threadIdx.x;
-      hops[j] = 0;
-      if (r_s[x[j]] != kOutside) live |= 1u << j;
-    }
-    while (live) {
+  for (int j = 0; j < kPer; j++) {
+    x[j] = j * BLOCK + threadIdx.x;
+    hops[j] = 0;
+    if (r_s[x[j]] != kOutside) live |= 1u << j;
+  }
+  while (live) {
 #pragma unroll
-      for (int j = 0; j < kWalkers; j++) {
-        if (!(live & (1u << j))) continue;
-        atomicAdd(&c_s[x[j]], 1u);
-        const int r = r_s[x[j]];
-        if (r >= 8) {
-          if (r == kNoFlow) rooted |= 1u << j;
-          live &= ~(1u << j);
-        } else if (++hops[j] > kHalo) {
-          live &= ~(1u << j);
-        } else {
-          x[j] += region_offset(r);
-        }
-      }
-    }
-    // the level of a tile cell is the length of its walk
-#pragma unroll
-    for (int j = 0; j < kWalkers; j++) {
-      const int i = (g + j) * BLOCK + threadIdx.x;
-      const int tx = i % kReg - kHalo, ty = i / kReg - kHalo;
-      if (r_s[i] == kOutside || tx < 0 || tx >= kTile || ty < 0 || ty >= kTile) continue;
-      const size_t gi = static_cast<size_t>(y0 + ty) * W + (x0 + tx);
-      if (rooted & (1u << j)) {
-        lvl[gi] = static_cast<uint8_t>(hops[j]);
-        if (level_hist) atomicAdd(&sh.hist[hops[j]], 1);
+    for (int j = 0; j < kPer; j++) {
+      if (!(live & (1u << j))) continue;
+      atomicAdd(&c_s[x[j]], 1u);
+      const int r = r_s[x[j]];
+      if (r >= 8) {
+        if (r == kNoFlow) rooted |= 1u << j;
+        live &= ~(1u << j);
+      } else if (++hops[j] > kHalo) {
+        live &= ~(1u << j);
       } else {
-        over = 1;
+        x[j] += region_offset(r);
       }
+    }
+  }
+  // the level of a cell is the length of its walk
+  int over = 0;
+#pragma unroll
+  for (int j = 0; j < kPer; j++) {
+    const int i = j * BLOCK + threadIdx.x;
+    const int tx = i % kReg - kHalo, ty = i / kReg - kHalo;
+    if (r_s[i] == kOutside || tx < 0 || tx >= kTile || ty < 0 || ty >= kTile) continue;
+    const size_t gi = static_cast<size_t>(y0 + ty) * W + (x0 + tx);
+    if (rooted & (1u << j)) {
+      lvl[gi] = static_cast<uint8_t>(hops[j]);
+      if (level_hist) atomicAdd(&sh.hist[hops[j]], 1);
+    } else {
+      over = 1;
     }
   }
   if (over) sh.over = 1;
   mark_halo_chains<BLOCK>(r_s, sh.mark);
   __syncthreads();
-  if (threadIdx.x < kRegCells / 32) {
-    const int c = __popc(sh.mark[threadIdx.x]);
-    if (c) atomicAdd(&sh.nmarked, c);
+  // halo slot bases: one wave scans the mark words, kMarkRun per lane
+  constexpr int kMarkWords = kRegCells / 32;
+  constexpr int kMarkRun = (kMarkWords + kWave - 1) / kWave;
+  if (threadIdx.x < kWave) {
+    const int w0 = threadIdx.x * kMarkRun;
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < kMarkRun; k++)
+      if (w0 + k < kMarkWords) v += __popc(sh.mark[w0 + k]);
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d);
+      if (static_cast<int>(threadIdx.x) >= d) x += y;
+    }
+    uint32_t run = x - v;
+#pragma unroll
+    for (int k = 0; k < kMarkRun; k++)
+      if (w0 + k < kMarkWords) {
+        sh.pre[w0 + k] = static_cast<uint16_t>(run);
+        run += __popc(sh.mark[w0 + k]);
+      }
+    if (threadIdx.x == kWave - 1) sh.scan[0] = x;
   }
   __syncthreads();
-  if (threadIdx.x == 0 && sh.nmarked > kHaloCap) sh.over = 1;
-  __syncthreads();
+  const uint32_t nhalo = sh.scan[0];
+  const int tile_over = sh.over || nhalo > static_cast<uint32_t>(kHaloCap);
   if (level_hist && threadIdx.x < kHalo + 1 && sh.hist[threadIdx.x])
     atomicAdd(&level_hist[threadIdx.x], sh.hist[threadIdx.x]);
-  for (int it = 0; it < kTile * kTile / BLOCK; it++) {
+  if (!tile_over) {
+    // the plan: every needed cell's receiver slot; halo cells' region index
+    // and level
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const int i = j * BLOCK + threadIdx.x;
+      if (!(sh.mark[i >> 5] & (1u << (i & 31)))) continue;
+      const int s = plan_slot(sh.mark, sh.pre, i);
+      plan->rslot[s] = plan_receiver(r_s, sh.mark, sh.pre, i);
+      plan->hreg[s - kTileCells] = static_cast<uint16_t>(i);
+      plan->hlvl[s - kTileCells] = static_cast<uint8_t>(hops[j]);
+    }
+    for (int it = 0; it < kTileCells / BLOCK; it++) {
+      const int tj = it * BLOCK + threadIdx.x;
+      const int i = (tj / kTile + kHalo) * kReg + tj % kTile + kHalo;
+      plan->rslot[tj] = r_s[i] == kOutside ? kPlanRoot : plan_receiver(r_s, sh.mark, sh.pre, i);
+    }
+    if (threadIdx.x == 0) plan->nhalo = static_cast<int32_t>(nhalo);
+  }
+  for (int it = 0; it < kTileCells / BLOCK; it++) {
     const int i = it * BLOCK + threadIdx.x;
     const int tx = i % kTile, ty = i / kTile;
     const int gx = x0 + tx, gy = y0 + ty;
     if (gx >= W || gy >= H) continue;
     cnt[static_cast<size_t>(gy) * W + gx] = c_s[(ty + kHalo) * kReg + tx + kHalo];
   }
-  return sh.over;
+  return tile_over;
 }
 
 __global__ __launch_bounds__(kAccBlock) void k_tile_acc(const uint8_t *__restrict__ rcv,
                                                         uint32_t *__restrict__ cnt,
-                                                        uint8_t *__restrict__ lvl, int32_t *hdr,
+                                                        uint8_t *__restrict__ lvl,
+                                                        TilePlan *__restrict__ plan, int32_t *hdr,
                                                         int W, int H, int tiles_x, int ntiles) {
   __shared__ AccShared sh;
-  const int over = tile_acc_body<kAccBlock>(sh, rcv, cnt, lvl, W, H,
-                                            xcd_tile(blockIdx.x, ntiles), tiles_x, nullptr);
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  const int over = tile_acc_body<kAccBlock>(sh, rcv, cnt, lvl, plan + t, W, H, t, tiles_x, nullptr);
   if (threadIdx.x == 0 && over) atomicAdd(&hdr[3], 1);
 }
 
 // Pass 2: uplift and erosion (AddUplift RB:285-296, Erode RB:306-328) of the
-// tile's cells.  Levels and counts come from pass 1.  Only the tile's cells
-// and the halo cells on their receiver chains are solved: mark_halo_chains
-// finds the latter (at most kHaloCap, checked in pass 1; the chains lie in
-// the region because every level is at most kHalo).  Cells live in slots --
-// tile cells at their tile index, marked halo cells after them in region
-// order, a halo slot being the rank of its mark bit -- so LDS holds 79 KB and
-// two tiles share a CU.  The needed cells are bucketed by level into a list
-// whose level segments are padded to whole 64-entry chunks; each wave takes
-// chunks in increasing order and each lane solves its cell with the
-// reference's Newton iteration as soon as its receiver (one level lower,
-// hence in an earlier chunk) is marked done.  With one level per chunk no
-// lane waits on a lane of its own wave, and the lowest chunk in progress
-// never waits on an unfinished one, so the wavefront always advances; the
-// wait is bounded all the same and reported in hdr[2].  Reads h_old, writes
-// h_new: neighbouring tiles read this tile's cells as their halo.  Callers
-// that run several tiles per workgroup put a barrier between calls.
+// tile's cells.  Counts and levels come from pass 1, and so does the tile's
+// plan: which halo cells lie on the receiver chains of the tile's cells (at
+// most kHaloCap; the chains lie in the region because every level is at most
+// kHalo) and every needed cell's receiver slot.  Only the tile's cells and
+// those halo cells are solved, in slots, so LDS holds 77 KB and two tiles
+// share a CU.  The needed cells are bucketed by level into a list whose
+// level segments are padded to whole 64-entry chunks; each wave takes chunks
+// in increasing order and each lane solves its cell with the reference's
+// Newton iteration as soon as its receiver (one level lower, hence in an
+// earlier chunk) is marked done.  With one level per chunk no lane waits on a
+// lane of its own wave, and the lowest chunk in progress never waits on an
+// unfinished one, so the wavefront always advances; the wait is bounded all
+// the same and reported in hdr[2].  Reads h_old, writes h_new: neighbouring
+// tiles read this tile's cells as their halo.  Callers that run several
+// tiles per workgroup put a barrier between calls.
 constexpr uint8_t kDone = 0xFD;
-constexpr int kTileCells = kTile * kTile;
-constexpr int kSlots = kTileCells + kHaloCap;
 constexpr int kListCap = kSlots + (kHalo + 1) * (kWave - 1);
 constexpr int kEroFactCache = 64;             // erosion factors cached in LDS
 struct EroShared {
@@ -876,27 +949,18 @@ struct EroShared {
   uint32_t c_s[kSlots];
   double ft_s[2 * kEroFactCache];
   uint16_t list[kListCap];
+  uint16_t rs_s[kSlots];        // receiver slot (plan rslot)
   uint16_t hreg[kHaloCap];      // region index of each halo slot
-  uint16_t pre[kRegCells / 32]; // marks before each mark word
-  uint8_t r_s[kRegCells];
   uint8_t d_s[kSlots];          // level, then kDone once solved
-  uint32_t mark[kRegCells / 32];
   uint32_t scan[kEroBlock / kWave + 1];
   int wcount[kHalo + 1][kEroBlock / kWave];
   int ltotal[kHalo + 1], loff[kHalo + 2];
 };
 
-// slot of region cell c (a tile cell, or a marked halo cell)
-__device__ __forceinline__ int ero_slot(const EroShared &sh, int c) {
-  const int lx = c % kReg, ly = c / kReg;
-  if (in_tile(lx, ly)) return (ly - kHalo) * kTile + (lx - kHalo);
-  return kTileCells + sh.pre[c >> 5] + __popc(sh.mark[c >> 5] & ((1u << (c & 31)) - 1u));
-}
-
 template <int NMODE>
 __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__restrict__ h_old,
                                                 double *__restrict__ h_new,
-                                                const uint8_t *__restrict__ rcv,
+                                                const TilePlan *__restrict__ plan,
                                                 const uint8_t *__restrict__ lvl,
                                                 const uint32_t *__restrict__ cnt,
                                                 const double *__restrict__ ftab, int W, int H,
@@ -904,37 +968,31 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
                                                 int32_t *hdr) {
   constexpr int kWaves = kEroBlock / kWave;
   constexpr int kKeys = kHalo + 1;
-  constexpr int kPer = kRegCells / kEroBlock;
   constexpr int kTilePer = kTileCells / kEroBlock;
   constexpr int kSlotPer = (kSlots + kEroBlock - 1) / kEroBlock;
   static_assert(kKeys <= 32, "level keys must fit the 5-bit match");
-  static_assert(kRegCells / 32 <= kEroBlock && 2 * kEroFactCache <= kEroBlock, "block shape");
+  static_assert(kHaloCap <= kEroBlock && 2 * kEroFactCache <= kEroBlock, "block shape");
   static_assert(kEroFactCache <= kFactCache, "factor table holds the cached counts");
   double *hs = sh.hs, *ft_s = sh.ft_s;
   uint32_t *c_s = sh.c_s;
-  uint16_t *list = sh.list;
-  uint8_t *r_s = sh.r_s, *d_s = sh.d_s;
+  uint16_t *list = sh.list, *rs_s = sh.rs_s;
+  uint8_t *d_s = sh.d_s;
   int(*wcount)[kWaves] = sh.wcount;
   int *ltotal = sh.ltotal, *loff = sh.loff;
   const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
   const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   for (int i = threadIdx.x; i < kKeys * kWaves; i += kEroBlock) (&wcount[0][0])[i] = 0;
-  if (threadIdx.x < kRegCells / 32) sh.mark[threadIdx.x] = 0;
   if (threadIdx.x < 2 * kEroFactCache) ft_s[threadIdx.x] = ftab[threadIdx.x];
-  // receivers of the whole region; elevation, count and level of the tile's
-  // cells
+  // the plan and the tile's elevations, counts and levels; a halo cell's
+  // elevation and count as soon as its region index arrives
+  const int nhalo = plan->nhalo;
+  const int nh = nhalo < 0 ? 0 : (nhalo > kHaloCap ? kHaloCap : nhalo);
   {
-    uint8_t rv[kPer], tl[kTilePer];
+    uint8_t tl[kTilePer];
     double th[kTilePer];
     uint32_t tc[kTilePer];
-#pragma unroll
-    for (int q = 0; q < kPer; q++) {
-      const int i = q * kEroBlock + threadIdx.x;
-      const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
-      const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
-      rv[q] = in ? rcv[static_cast<size_t>(gy) * W + gx] : kOutside;
-    }
+    uint16_t rv[kSlotPer];
 #pragma unroll
     for (int j = 0; j < kTilePer; j++) {
       const int tj = j * kEroBlock + threadIdx.x;
@@ -946,86 +1004,42 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
       tl[j] = in ? lvl[g] : kEscaped;
     }
 #pragma unroll
-    for (int q = 0; q < kPer; q++) r_s[q * kEroBlock + threadIdx.x] = rv[q];
+    for (int q = 0; q < kSlotPer; q++) {
+      const int sl = q * kEroBlock + threadIdx.x;
+      rv[q] = sl < kTileCells + nh ? plan->rslot[sl] : kPlanRoot;
+    }
+    const bool halo = threadIdx.x < nh;
+    const int hr = halo ? plan->hreg[threadIdx.x] : 0;
+    const uint8_t hl = halo ? plan->hlvl[threadIdx.x] : 0;
+    const size_t hg = static_cast<size_t>(ry0 + hr / kReg) * W + (rx0 + hr % kReg);
+    const double hh = halo ? h_old[hg] : 0.0;
+    const uint32_t hc = halo ? cnt[hg] : 0u;
 #pragma unroll
     for (int j = 0; j < kTilePer; j++) {
       hs[j * kEroBlock + threadIdx.x] = th[j];
       c_s[j * kEroBlock + threadIdx.x] = tc[j];
       d_s[j * kEroBlock + threadIdx.x] = tl[j];
     }
+#pragma unroll
+    for (int q = 0; q < kSlotPer; q++) {
+      const int sl = q * kEroBlock + threadIdx.x;
+      if (sl < kSlots) rs_s[sl] = rv[q];
+    }
+    if (halo) {
+      hs[kTileCells + threadIdx.x] = hh;
+      c_s[kTileCells + threadIdx.x] = hc;
+      d_s[kTileCells + threadIdx.x] = hl;
+      sh.hreg[threadIdx.x] = static_cast<uint16_t>(hr);
+    }
   }
-  __syncthreads();
-  mark_halo_chains<kEroBlock>(r_s, sh.mark);
-  __syncthreads();
-  uint32_t nhalo;
-  {
-    const uint32_t v = threadIdx.x < kRegCells / 32 ? __popc(sh.mark[threadIdx.x]) : 0u;
-    const uint32_t ex = block_excl_scan<kEroBlock>(v, &nhalo, sh.scan);
-    if (threadIdx.x < kRegCells / 32) sh.pre[threadIdx.x] = static_cast<uint16_t>(ex);
-  }
-  if (nhalo > static_cast<uint32_t>(kHaloCap)) {   // excluded by pass 1
+  if (nh != nhalo) {   // excluded by pass 1
     if (threadIdx.x == 0) atomicOr(&hdr[2], kErrIndex);
     return;
   }
   __syncthreads();
-  // elevation and level of the marked halo cells: a thread's first two are
-  // loaded together, any further ones after
-  {
-    int ia = -1, ib = -1;
-    uint32_t rest = 0;
-#pragma unroll
-    for (int q = 0; q < kPer; q++) {
-      const int i = q * kEroBlock + threadIdx.x;
-      if (!(sh.mark[i >> 5] & (1u << (i & 31)))) continue;
-      if (ia < 0) ia = i;
-      else if (ib < 0) ib = i;
-      else rest |= 1u << q;
-    }
-    auto gidx = [&](int i) {
-      return static_cast<size_t>(ry0 + i / kReg) * W + (rx0 + i % kReg);
-    };
-    double ha = 0.0, hb = 0.0;
-    uint32_t ca = 0, cb = 0;
-    uint8_t da = 0, db = 0;
-    if (ia >= 0) {
-      ha = h_old[gidx(ia)];
-      ca = cnt[gidx(ia)];
-      da = lvl[gidx(ia)];
-    }
-    if (ib >= 0) {
-      hb = h_old[gidx(ib)];
-      cb = cnt[gidx(ib)];
-      db = lvl[gidx(ib)];
-    }
-    if (ia >= 0) {
-      const int sl = ero_slot(sh, ia);
-      hs[sl] = ha;
-      c_s[sl] = ca;
-      d_s[sl] = da;
-      sh.hreg[sl - kTileCells] = static_cast<uint16_t>(ia);
-    }
-    if (ib >= 0) {
-      const int sl = ero_slot(sh, ib);
-      hs[sl] = hb;
-      c_s[sl] = cb;
-      d_s[sl] = db;
-      sh.hreg[sl - kTileCells] = static_cast<uint16_t>(ib);
-    }
-    while (rest) {
-      const int q = __builtin_ctz(rest);
-      rest &= rest - 1;
-      const int i = q * kEroBlock + threadIdx.x;
-      const int sl = ero_slot(sh, i);
-      hs[sl] = h_old[gidx(i)];
-      c_s[sl] = cnt[gidx(i)];
-      d_s[sl] = lvl[gidx(i)];
-      sh.hreg[sl - kTileCells] = static_cast<uint16_t>(i);
-    }
-  }
-  __syncthreads();
   // bucket the slots by level: rank within (level, wave) by matching keys
   // with five ballots, no atomics; then level bases
-  const int nslots = kTileCells + static_cast<int>(nhalo);
+  const int nslots = kTileCells + nh;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
   int rank[kSlotPer];
 #pragma unroll
@@ -1079,18 +1093,22 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
   for (int ch = wave; ch < nchunks; ch += kWaves) {
     const int sl = list[ch * kWave + lane];
     if (sl == 0xFFFF) continue;
-    const int i = sl < kTileCells
-                      ? (sl / kTile + kHalo) * kReg + sl % kTile + kHalo
-                      : sh.hreg[sl - kTileCells];
-    int lx = i % kReg, ly = i / kReg;
-    const int gx = rx0 + lx, gy = ry0 + ly;
     if (ch < root_chunks) {
       // level 0: uplift of interior outlets; rings 0 and 1 stay fixed
+      int gx, gy;
+      if (sl < kTileCells) {
+        gx = x0 + sl % kTile;
+        gy = y0 + sl / kTile;
+      } else {
+        const int i = sh.hreg[sl - kTileCells];
+        gx = rx0 + i % kReg;
+        gy = ry0 + i / kReg;
+      }
       if (gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2) hs[sl] += k.ud;
     } else {
-      const int r = r_s[i];
-      if (region_step(r, lx, ly)) {          // always, for a cell on a tile chain
-        const int nb = ero_slot(sh, ly * kReg + lx);
+      const int rs = rs_s[sl];
+      const int nb = rs & kPlanSlot;
+      if (nb < nslots) {                      // always, for a cell above level 0
         int spins = 0;
         while (__hip_atomic_load(&d_s[nb], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) !=
                kDone) {
@@ -1101,9 +1119,11 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
           __builtin_amdgcn_s_sleep(1);
         }
         const uint32_t a = c_s[sl];
-        const int ti = 2 * static_cast<int>(a) + (r & 1);
+        const int ti = 2 * static_cast<int>(a) + (rs >> 15);
         const double fact = a < kEroFactCache ? ft_s[ti] : ftab[ti];
         hs[sl] = newton_update<NMODE>(hs[sl] + k.ud, hs[nb], fact, k, hdr);
+      } else {
+        atomicOr(&hdr[2], kErrIndex);
       }
     }
     __hip_atomic_store(&d_s[sl], kDone, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1121,12 +1141,12 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
 template <int NMODE>
 __global__ __launch_bounds__(kEroBlock) void k_tile_erode(
     const double *__restrict__ h_old, double *__restrict__ h_new,
-    const uint8_t *__restrict__ rcv, const uint8_t *__restrict__ lvl,
+    const TilePlan *__restrict__ plan, const uint8_t *__restrict__ lvl,
     const uint32_t *__restrict__ cnt, const double *__restrict__ ftab, int W, int H,
     int tiles_x, int ntiles, ErodeConst k, int32_t *hdr) {
   __shared__ EroShared sh;
-  tile_erode_body<NMODE>(sh, h_old, h_new, rcv, lvl, cnt, ftab, W, H, xcd_tile(blockIdx.x, ntiles),
-                         tiles_x, k, hdr);
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  tile_erode_body<NMODE>(sh, h_old, h_new, plan + t, lvl, cnt, ftab, W, H, t, tiles_x, k, hdr);
 }
 
 }  // namespace

@@ -77,6 +77,7 @@ struct EnsReal {
   uint64_t off;        // first cell of the realization in the cell arrays
   uint64_t ftab_off;   // first entry of its erosion factor table
   uint64_t lev_off;    // first entry of its level offsets
+  uint64_t plan_off;   // first tile plan of the realization
   int W, H, tiles_x, ntiles;
   int lev_cap;         // level offsets available (deepest level + 3)
   int nmode;
@@ -93,6 +94,7 @@ struct EnsArgs {
   double *h_old, *h_new;
   uint8_t *rcv, *lvl, *dmask;
   uint32_t *cnt;
+  TilePlan *plan;
   int32_t *order, *lev;
   const double *ftab;
   int32_t *rh, *hist, *ctl;
@@ -421,9 +423,10 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_step(EnsArgs a) {
       const int e = find_entry(a.tabA_off, entries + 1, item);
       const int r = a.tabA_real[e];
       const EnsReal &g = a.real[r];
+      const int t = item - a.tabA_off[e];
       const int over = tile_acc_body<kEnsBlock>(
-          sh.acc, a.rcv + g.off, a.cnt + g.off, a.lvl + g.off, g.W, g.H, item - a.tabA_off[e],
-          g.tiles_x, a.hist + static_cast<size_t>(r) * kHistLen);
+          sh.acc, a.rcv + g.off, a.cnt + g.off, a.lvl + g.off, a.plan + g.plan_off + t, g.W, g.H,
+          t, g.tiles_x, a.hist + static_cast<size_t>(r) * kHistLen);
       if (threadIdx.x == 0 && over) atomicAdd(&a.rh[static_cast<size_t>(r) * kRH + kRhOver], 1);
       __syncthreads();
     }
@@ -492,7 +495,8 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_step(EnsArgs a) {
       for (int tile = j * a.tiles_per_claim; tile < t1; tile++) {
         with_nmode<NM>(g.nmode, [&](auto c) {
           tile_erode_body<decltype(c)::value>(sh.ero, a.h_old + g.off, a.h_new + g.off,
-                                              a.rcv + g.off, a.lvl + g.off, a.cnt + g.off,
+                                              a.plan + g.plan_off + tile, a.lvl + g.off,
+                                              a.cnt + g.off,
                                               a.ftab + g.ftab_off, g.W, g.H, tile, g.tiles_x,
                                               ek, q);
         });
@@ -525,6 +529,7 @@ struct fs_ensemble {
   double *h[2] = {nullptr, nullptr};
   uint8_t *rcv = nullptr, *lvl = nullptr, *dmask = nullptr;
   uint32_t *cnt = nullptr;
+  TilePlan *plan = nullptr;
   int32_t *order = nullptr, *lev = nullptr;
   double *ftab = nullptr;
   int32_t *rh = nullptr, *hist = nullptr, *ctl = nullptr;
@@ -553,7 +558,8 @@ void free_ensemble(fs_ensemble *e) {
   for (void *p : {static_cast<void *>(e->real_d), static_cast<void *>(e->h[0]),
                   static_cast<void *>(e->h[1]), static_cast<void *>(e->rcv),
                   static_cast<void *>(e->lvl), static_cast<void *>(e->dmask),
-                  static_cast<void *>(e->cnt), static_cast<void *>(e->order),
+                  static_cast<void *>(e->cnt), static_cast<void *>(e->plan),
+                  static_cast<void *>(e->order),
                   static_cast<void *>(e->lev), static_cast<void *>(e->ftab),
                   static_cast<void *>(e->rh), static_cast<void *>(e->hist),
                   static_cast<void *>(e->ctl), static_cast<void *>(e->tabA_off),
@@ -634,6 +640,7 @@ int launch_step(fs_ensemble *e) {
   a.lvl = e->lvl;
   a.dmask = e->dmask;
   a.cnt = e->cnt;
+  a.plan = e->plan;
   a.order = e->order;
   a.lev = e->lev;
   a.ftab = e->ftab;
@@ -863,7 +870,10 @@ int fs_ensemble_create(const fs_realization *reals, int count, int device, fs_en
   e->nm_all = nm0 == 0 ? -1 : nm0;
   e->cells = cells;
   long long tiles_total = 0;
-  for (const EnsReal &g : e->real) tiles_total += g.ntiles;
+  for (EnsReal &g : e->real) {
+    g.plan_off = static_cast<uint64_t>(tiles_total);
+    tiles_total += g.ntiles;
+  }
   if (tiles_total >= (1ll << 31)) return FS_ERR_DIMENSIONS;
 
   FS_HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
@@ -875,6 +885,7 @@ int fs_ensemble_create(const fs_realization *reals, int count, int device, fs_en
   FS_TRY(fs::dev_alloc(&e->lvl, cells));
   FS_TRY(fs::dev_alloc(&e->dmask, cells));
   FS_TRY(fs::dev_alloc(&e->cnt, cells));
+  FS_TRY(fs::dev_alloc(&e->plan, static_cast<size_t>(tiles_total)));
   FS_TRY(fs::dev_alloc(&e->order, cells));
   FS_TRY(fs::dev_alloc(&e->lev, lev_n));
   FS_TRY(fs::dev_alloc(&e->ftab, ftab_n));
