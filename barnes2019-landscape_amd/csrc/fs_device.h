@@ -809,9 +809,10 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
   load_region_receivers<BLOCK>(rcv, r_s, rx0, ry0, W, H);
   __syncthreads();
   // Every region cell walks its receiver chain and adds 1 to each cell it
-  // passes, itself included; the thread's chains advance together so that
-  // their dependent LDS reads overlap.  A walker stops at an outlet, where
-  // its chain leaves the region, or after kHalo hops.
+  // passes, itself included; the thread's chains advance together, one cell
+  // per round.  A walker stops at an outlet (its level is the round it got
+  // there), where its chain leaves the region, or after kHalo hops (kHalo + 1
+  // rounds).
   constexpr int kPer = kRegCells / BLOCK;
   static_assert(kPer <= 16, "walkers per thread");
   int x[kPer], hops[kPer];
@@ -822,19 +823,20 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
     hops[j] = 0;
     if (r_s[x[j]] != kOutside) live |= 1u << j;
   }
-  while (live) {
+  for (int round = 0; round <= kHalo && live; round++) {
 #pragma unroll
     for (int j = 0; j < kPer; j++) {
       if (!(live & (1u << j))) continue;
       atomicAdd(&c_s[x[j]], 1u);
       const int r = r_s[x[j]];
-      if (r >= 8) {
-        if (r == kNoFlow) rooted |= 1u << j;
-        live &= ~(1u << j);
-      } else if (++hops[j] > kHalo) {
-        live &= ~(1u << j);
-      } else {
+      if (r < 8) {
         x[j] += region_offset(r);
+      } else {
+        if (r == kNoFlow) {
+          rooted |= 1u << j;
+          hops[j] = round;
+        }
+        live &= ~(1u << j);
       }
     }
   }
