@@ -643,7 +643,6 @@ constexpr int kAccBlock = 1024;
 constexpr int kEroBlock = 1024;
 constexpr uint8_t kOutside = 0xFD;            // region cell beyond the grid
 constexpr uint8_t kEscaped = 0xFF;            // chain leaves the region
-constexpr uint8_t kLeaves = 0xFE;             // receiver lies outside the region
 constexpr int kFactCache = 256;               // erosion factors cached in LDS
 
 // Region-local neighbour step in direction r; false if it leaves the region.
@@ -680,34 +679,6 @@ __device__ __forceinline__ int region_offset(int r) {
   return static_cast<int8_t>(static_cast<uint8_t>(kPacked >> (8 * r)));
 }
 
-// Load the receivers of the region into r_s (every load of a thread is
-// issued before the first LDS store).  A receiver outside the region is
-// stored as kLeaves, so walks over r_s follow region_offset without bounds
-// checks: they stop at any code >= 8 (kNoFlow: an outlet; kLeaves: the chain
-// leaves the region; kOutside: beyond the grid, never reached by a walk).
-template <int BLOCK>
-__device__ __forceinline__ void load_region_receivers(const uint8_t *__restrict__ rcv,
-                                                      uint8_t *r_s, int rx0, int ry0, int W,
-                                                      int H) {
-  constexpr int kPer = kRegCells / BLOCK;
-  static_assert(kRegCells % BLOCK == 0, "region must split evenly over the block");
-  uint8_t rv[kPer];
-#pragma unroll
-  for (int k = 0; k < kPer; k++) {
-    const int i = k * BLOCK + threadIdx.x;
-    const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
-    const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
-    rv[k] = in ? rcv[static_cast<size_t>(gy) * W + gx] : kOutside;
-  }
-#pragma unroll
-  for (int k = 0; k < kPer; k++) {
-    const int i = k * BLOCK + threadIdx.x;
-    int lx = i % kReg, ly = i / kReg;
-    const int r = rv[k];
-    r_s[i] = (r < 8 && !region_step(r, lx, ly)) ? kLeaves : rv[k];
-  }
-}
-
 // Pass 1: upstream cell counts (ComputeFlowAcc, RB:262-279) and levels
 // (receiver hops to an outlet, the level GenerateOrder assigns, RB:213-254)
 // of the tile's cells.  Every region cell walks its receiver chain once and
@@ -724,22 +695,73 @@ __device__ __forceinline__ bool in_tile(int lx, int ly) {
   return lx >= kHalo && lx < kHalo + kTile && ly >= kHalo && ly < kHalo + kTile;
 }
 
+// Accumulation word of a region cell in LDS: bits 0-13 its count, bit 23
+// set for a tile cell, bits 24-31 the signed region-index offset of its
+// receiver or a stop code.  One atomic add both counts a walker's visit and
+// tells it where to go next.
+constexpr uint32_t kAccCount = 0x3FFFu;
+constexpr uint32_t kAccTile = 1u << 23;
+constexpr int kStopRoot = -128;      // an outlet (NO_FLOW)
+constexpr int kStopLeaves = -127;    // the receiver lies outside the region
+constexpr int kStopOutside = -126;   // beyond the grid
+constexpr int kStopMax = -126;       // codes <= kStopMax end a walk
+static_assert(kRegCells <= static_cast<int>(kAccCount) && kReg + 1 < -kStopMax,
+              "accumulation word layout");
+
+__device__ __forceinline__ int acc_code(uint32_t w) { return static_cast<int32_t>(w) >> 24; }
+
+// Load the receivers of the region into accumulation words with zero counts
+// (every load of a thread is issued before the first LDS store).
+template <int BLOCK>
+__device__ __forceinline__ void load_region_words(const uint8_t *__restrict__ rcv,
+                                                  uint32_t *w_s, int rx0, int ry0, int W,
+                                                  int H) {
+  constexpr int kPer = kRegCells / BLOCK;
+  static_assert(kRegCells % BLOCK == 0, "region must split evenly over the block");
+  uint8_t rv[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const int i = k * BLOCK + threadIdx.x;
+    const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
+    const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
+    rv[k] = in ? rcv[static_cast<size_t>(gy) * W + gx] : kOutside;
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const int i = k * BLOCK + threadIdx.x;
+    const int lx = i % kReg, ly = i / kReg;
+    const int r = rv[k];
+    int code;
+    if (r == kOutside) {
+      code = kStopOutside;
+    } else if (r >= 8) {
+      code = kStopRoot;
+    } else {
+      int nx = lx, ny = ly;
+      code = region_step(r, nx, ny) ? region_offset(r) : kStopLeaves;
+    }
+    w_s[i] = static_cast<uint32_t>(code & 0xFF) << 24 | (in_tile(lx, ly) ? kAccTile : 0u);
+  }
+}
+
 // Mark (bitmask over region cells) the halo cells on the receiver chains of
 // the tile's cells: walks start at tile cells whose receiver leaves the tile
 // and stop at an outlet, the region edge, a tile cell, an already marked cell
-// or after kHalo hops.  mark must be zero and r_s loaded before the call.
+// or after kHalo hops.  mark must be zero and w_s loaded before the call
+// (counts may still be changing: only the top byte and the tile bit are read).
 template <int BLOCK>
-__device__ __forceinline__ void mark_halo_chains(const uint8_t *r_s, uint32_t *mark) {
+__device__ __forceinline__ void mark_halo_chains(const uint32_t *w_s, uint32_t *mark) {
   for (int it = 0; it < kTile * kTile / BLOCK; it++) {
     const int j = it * BLOCK + threadIdx.x;
-    int lx = kHalo + j % kTile, ly = kHalo + j / kTile;
-    int r = r_s[ly * kReg + lx];
-    for (int hop = 0; hop <= kHalo && r < 8; hop++) {
-      if (!region_step(r, lx, ly) || in_tile(lx, ly)) break;
-      const int c = ly * kReg + lx;
+    int c = (kHalo + j / kTile) * kReg + kHalo + j % kTile;
+    int code = acc_code(w_s[c]);
+    for (int hop = 0; hop <= kHalo && code > kStopMax; hop++) {
+      c += code;
+      const uint32_t wd = w_s[c];
+      if (wd & kAccTile) break;
       const uint32_t bit = 1u << (c & 31);
       if (atomicOr(&mark[c >> 5], bit) & bit) break;   // marked by another walk
-      r = r_s[c];
+      code = acc_code(wd);
     }
   }
 }
@@ -762,8 +784,7 @@ struct alignas(16) TilePlan {
 };
 
 struct AccShared {
-  uint8_t r_s[kRegCells];
-  uint32_t c_s[kRegCells];
+  uint32_t w_s[kRegCells];        // accumulation words
   uint32_t mark[kRegCells / 32];
   uint16_t pre[kRegCells / 32];   // marks before each mark word
   uint32_t scan[1];               // number of marks
@@ -779,12 +800,13 @@ __device__ __forceinline__ int plan_slot(const uint32_t *mark, const uint16_t *p
 }
 
 // rslot entry of region cell c (a tile cell or a marked halo cell).
-__device__ __forceinline__ uint16_t plan_receiver(const uint8_t *r_s, const uint32_t *mark,
+__device__ __forceinline__ uint16_t plan_receiver(const uint32_t *w_s, const uint32_t *mark,
                                                   const uint16_t *pre, int c) {
-  const int r = r_s[c];
-  if (r >= 8) return kPlanRoot;   // an outlet (a receiver beyond the region makes the tile over)
-  const int s = plan_slot(mark, pre, c + region_offset(r));
-  return static_cast<uint16_t>(s | ((r & 1) ? kPlanDiag : 0u));
+  const int code = acc_code(w_s[c]);
+  if (code <= kStopMax) return kPlanRoot;   // an outlet (leaving the region makes the tile over)
+  const int s = plan_slot(mark, pre, c + code);
+  const bool diag = code != 1 && code != -1 && code != kReg && code != -kReg;
+  return static_cast<uint16_t>(s | (diag ? kPlanDiag : 0u));
 }
 
 // Returns nonzero (block-uniform) when a tile cell is deeper than kHalo or
@@ -798,41 +820,40 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
                                              TilePlan *__restrict__ plan, int W, int H, int t,
                                              int tiles_x, int32_t *level_hist) {
   static_assert(BLOCK == kAccBlock, "shared layout is sized for kAccBlock");
-  uint8_t *r_s = sh.r_s;
-  uint32_t *c_s = sh.c_s;
+  uint32_t *w_s = sh.w_s;
   const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
   const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
   if (threadIdx.x == 0) sh.over = 0;
   if (threadIdx.x < kHalo + 1) sh.hist[threadIdx.x] = 0;
   if (threadIdx.x < kRegCells / 32) sh.mark[threadIdx.x] = 0;
-  for (int it = 0; it < kRegCells / BLOCK; it++) c_s[it * BLOCK + threadIdx.x] = 0;
-  load_region_receivers<BLOCK>(rcv, r_s, rx0, ry0, W, H);
+  load_region_words<BLOCK>(rcv, w_s, rx0, ry0, W, H);
   __syncthreads();
   // Every region cell walks its receiver chain and adds 1 to each cell it
   // passes, itself included; the thread's chains advance together, one cell
   // per round.  A walker stops at an outlet (its level is the round it got
   // there), where its chain leaves the region, or after kHalo hops (kHalo + 1
-  // rounds).
+  // rounds).  Walkers hold byte addresses into w_s.
   constexpr int kPer = kRegCells / BLOCK;
   static_assert(kPer <= 16, "walkers per thread");
-  int x[kPer], hops[kPer];
+  int xa[kPer], hops[kPer];
   uint32_t live = 0, rooted = 0;
 #pragma unroll
   for (int j = 0; j < kPer; j++) {
-    x[j] = j * BLOCK + threadIdx.x;
+    const int i = j * BLOCK + threadIdx.x;
+    xa[j] = 4 * i;
     hops[j] = 0;
-    if (r_s[x[j]] != kOutside) live |= 1u << j;
+    if (acc_code(w_s[i]) != kStopOutside) live |= 1u << j;
   }
   for (int round = 0; round <= kHalo && live; round++) {
 #pragma unroll
     for (int j = 0; j < kPer; j++) {
       if (!(live & (1u << j))) continue;
-      atomicAdd(&c_s[x[j]], 1u);
-      const int r = r_s[x[j]];
-      if (r < 8) {
-        x[j] += region_offset(r);
+      uint32_t *w = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(w_s) + xa[j]);
+      const int code = acc_code(atomicAdd(w, 1u));
+      if (code > kStopMax) {
+        xa[j] += 4 * code;
       } else {
-        if (r == kNoFlow) {
+        if (code == kStopRoot) {
           rooted |= 1u << j;
           hops[j] = round;
         }
@@ -846,7 +867,8 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
   for (int j = 0; j < kPer; j++) {
     const int i = j * BLOCK + threadIdx.x;
     const int tx = i % kReg - kHalo, ty = i / kReg - kHalo;
-    if (r_s[i] == kOutside || tx < 0 || tx >= kTile || ty < 0 || ty >= kTile) continue;
+    if (tx < 0 || tx >= kTile || ty < 0 || ty >= kTile) continue;
+    if (acc_code(w_s[i]) == kStopOutside) continue;
     const size_t gi = static_cast<size_t>(y0 + ty) * W + (x0 + tx);
     if (rooted & (1u << j)) {
       lvl[gi] = static_cast<uint8_t>(hops[j]);
@@ -856,7 +878,7 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
     }
   }
   if (over) sh.over = 1;
-  mark_halo_chains<BLOCK>(r_s, sh.mark);
+  mark_halo_chains<BLOCK>(w_s, sh.mark);
   __syncthreads();
   // halo slot bases: one wave scans the mark words, kMarkRun per lane
   constexpr int kMarkWords = kRegCells / 32;
@@ -895,14 +917,14 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
       const int i = j * BLOCK + threadIdx.x;
       if (!(sh.mark[i >> 5] & (1u << (i & 31)))) continue;
       const int s = plan_slot(sh.mark, sh.pre, i);
-      plan->rslot[s] = plan_receiver(r_s, sh.mark, sh.pre, i);
+      plan->rslot[s] = plan_receiver(w_s, sh.mark, sh.pre, i);
       plan->hreg[s - kTileCells] = static_cast<uint16_t>(i);
       plan->hlvl[s - kTileCells] = static_cast<uint8_t>(hops[j]);
     }
     for (int it = 0; it < kTileCells / BLOCK; it++) {
       const int tj = it * BLOCK + threadIdx.x;
       const int i = (tj / kTile + kHalo) * kReg + tj % kTile + kHalo;
-      plan->rslot[tj] = r_s[i] == kOutside ? kPlanRoot : plan_receiver(r_s, sh.mark, sh.pre, i);
+      plan->rslot[tj] = plan_receiver(w_s, sh.mark, sh.pre, i);
     }
     if (threadIdx.x == 0) plan->nhalo = static_cast<int32_t>(nhalo);
   }
@@ -911,7 +933,7 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
     const int tx = i % kTile, ty = i / kTile;
     const int gx = x0 + tx, gy = y0 + ty;
     if (gx >= W || gy >= H) continue;
-    cnt[static_cast<size_t>(gy) * W + gx] = c_s[(ty + kHalo) * kReg + tx + kHalo];
+    cnt[static_cast<size_t>(gy) * W + gx] = w_s[(ty + kHalo) * kReg + tx + kHalo] & kAccCount;
   }
   return tile_over;
 }
