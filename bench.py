@@ -52,9 +52,9 @@ KERNEL_BYTES = {
 # profiles/tile_path_10000x10000_hbm_traffic.md.  Reported as `traffic` for
 # that workload only.
 MEASURED_TRAFFIC_PER_CELL = {
-    "k_tile_erode": 23.32,
-    "k_tile_acc": 6.20,
-    "k_receivers": 10.70,
+    "k_tile_erode": 23.89,
+    "k_tile_acc": 8.60,
+    "k_receivers": 10.69,
 }
 
 
