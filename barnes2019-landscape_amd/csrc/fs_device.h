@@ -718,18 +718,31 @@ __device__ __forceinline__ void load_region_words(const uint8_t *__restrict__ rc
                                                   int H) {
   constexpr int kPer = kRegCells / BLOCK;
   static_assert(kRegCells % BLOCK == 0, "region must split evenly over the block");
+  // a thread's cells are threadIdx.x + k * BLOCK: step the region column and
+  // row instead of dividing
+  constexpr int kStepY = BLOCK / kReg, kStepX = BLOCK % kReg;
+  const int lx0 = threadIdx.x % kReg, ly0 = threadIdx.x / kReg;
+  const bool inside = rx0 >= 0 && ry0 >= 0 && rx0 + kReg <= W && ry0 + kReg <= H;
   uint8_t rv[kPer];
+  {
+    int lx = lx0, ly = ly0;
 #pragma unroll
-  for (int k = 0; k < kPer; k++) {
-    const int i = k * BLOCK + threadIdx.x;
-    const int gx = rx0 + i % kReg, gy = ry0 + i / kReg;
-    const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
-    rv[k] = in ? rcv[static_cast<size_t>(gy) * W + gx] : kOutside;
+    for (int k = 0; k < kPer; k++) {
+      const int gx = rx0 + lx, gy = ry0 + ly;
+      const bool in = inside || (gx >= 0 && gx < W && gy >= 0 && gy < H);
+      rv[k] = in ? rcv[static_cast<size_t>(gy) * W + gx] : kOutside;
+      lx += kStepX;
+      ly += kStepY;
+      if (lx >= kReg) {
+        lx -= kReg;
+        ly++;
+      }
+    }
   }
+  int lx = lx0, ly = ly0;
 #pragma unroll
   for (int k = 0; k < kPer; k++) {
     const int i = k * BLOCK + threadIdx.x;
-    const int lx = i % kReg, ly = i / kReg;
     const int r = rv[k];
     int code;
     if (r == kOutside) {
@@ -741,6 +754,12 @@ __device__ __forceinline__ void load_region_words(const uint8_t *__restrict__ rc
       code = region_step(r, nx, ny) ? region_offset(r) : kStopLeaves;
     }
     w_s[i] = static_cast<uint32_t>(code & 0xFF) << 24 | (in_tile(lx, ly) ? kAccTile : 0u);
+    lx += kStepX;
+    ly += kStepY;
+    if (lx >= kReg) {
+      lx -= kReg;
+      ly++;
+    }
   }
 }
 
@@ -861,10 +880,13 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
       }
     }
   }
-  // the level of a cell is the length of its walk
+  // the level of a cell is the length of its walk; only slots whose region
+  // rows can meet the tile's rows hold tile cells
   int over = 0;
+  constexpr int kJ0 = (kHalo * kReg) / BLOCK;
+  constexpr int kJ1 = ((kHalo + kTile) * kReg - 1) / BLOCK;
 #pragma unroll
-  for (int j = 0; j < kPer; j++) {
+  for (int j = kJ0; j <= kJ1; j++) {
     const int i = j * BLOCK + threadIdx.x;
     const int tx = i % kReg - kHalo, ty = i / kReg - kHalo;
     if (tx < 0 || tx >= kTile || ty < 0 || ty >= kTile) continue;
@@ -915,16 +937,34 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
 #pragma unroll
     for (int j = 0; j < kPer; j++) {
       const int i = j * BLOCK + threadIdx.x;
-      if (!(sh.mark[i >> 5] & (1u << (i & 31)))) continue;
-      const int s = plan_slot(sh.mark, sh.pre, i);
+      const uint32_t mw = sh.mark[i >> 5];
+      if (!(mw & (1u << (i & 31)))) continue;
+      const int s = kTileCells + sh.pre[i >> 5] + __popc(mw & ((1u << (i & 31)) - 1u));
       plan->rslot[s] = plan_receiver(w_s, sh.mark, sh.pre, i);
       plan->hreg[s - kTileCells] = static_cast<uint16_t>(i);
       plan->hlvl[s - kTileCells] = static_cast<uint8_t>(hops[j]);
     }
     for (int it = 0; it < kTileCells / BLOCK; it++) {
       const int tj = it * BLOCK + threadIdx.x;
-      const int i = (tj / kTile + kHalo) * kReg + tj % kTile + kHalo;
-      plan->rslot[tj] = plan_receiver(w_s, sh.mark, sh.pre, i);
+      const int tx = tj % kTile, ty = tj / kTile;
+      const int i = (ty + kHalo) * kReg + tx + kHalo;
+      const int code = acc_code(w_s[i]);
+      uint16_t rs = kPlanRoot;
+      if (code > kStopMax) {
+        // code = dy * kReg + dx with dx, dy in {-1, 0, 1}
+        const int dy = (code > 1) - (code < -1);
+        const int dx = code - dy * kReg;
+        int sl;
+        if (static_cast<unsigned>(tx + dx) < static_cast<unsigned>(kTile) &&
+            static_cast<unsigned>(ty + dy) < static_cast<unsigned>(kTile)) {
+          sl = tj + dy * kTile + dx;
+        } else {
+          const int nb = i + code;
+          sl = kTileCells + sh.pre[nb >> 5] + __popc(sh.mark[nb >> 5] & ((1u << (nb & 31)) - 1u));
+        }
+        rs = static_cast<uint16_t>(sl | ((dx != 0 && dy != 0) ? kPlanDiag : 0u));
+      }
+      plan->rslot[tj] = rs;
     }
     if (threadIdx.x == 0) plan->nhalo = static_cast<int32_t>(nhalo);
   }
@@ -1074,8 +1114,10 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
     uint64_t m = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 5; b++) {
+      // keep the lanes whose bit b equals this lane's: ~(ballot ^ all-ones-if-set)
       const uint64_t bb = __ballot((d >> b) & 1);
-      m &= ((d >> b) & 1) ? bb : ~bb;
+      const uint64_t mine = static_cast<uint64_t>(-static_cast<int64_t>((d >> b) & 1));
+      m &= ~(bb ^ mine);
     }
     const int before = __popcll(m & lt);
     const int base = valid ? wcount[d][wave] : 0;
