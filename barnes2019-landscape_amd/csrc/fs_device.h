@@ -763,6 +763,72 @@ __device__ __forceinline__ void load_region_words(const uint8_t *__restrict__ rc
   }
 }
 
+// Bytes of a 4-entry table, first entry lowest.
+constexpr uint32_t pack4(int a, int b, int c, int d) {
+  return static_cast<uint32_t>(a & 0xFF) | static_cast<uint32_t>(b & 0xFF) << 8 |
+         static_cast<uint32_t>(c & 0xFF) << 16 | static_cast<uint32_t>(d & 0xFF) << 24;
+}
+
+// Same words as load_region_words, four cells per dword, for a region that
+// lies inside the grid with 4-byte aligned rows.  v_perm_b32 looks bytes up
+// in an 8-entry table (selector 0..7; 12 gives 0): one lookup turns four
+// receiver directions into four offsets, four more flag the directions that
+// leave the region through its first/last column and row.
+template <int BLOCK>
+__device__ __forceinline__ void load_region_words_x4(const uint8_t *__restrict__ rcv,
+                                                     uint32_t *w_s, int rx0, int ry0, int W) {
+  constexpr int kRowWords = kReg / 4;
+  constexpr int kWords = kRegCells / 4;
+  constexpr int kIters = (kWords + BLOCK - 1) / BLOCK;
+  static_assert(kReg % 4 == 0 && kHalo % 4 == 0 && kTile % 4 == 0, "dword cells");
+  // offsets of directions 0..3 (low table) and 4..7 (high table)
+  constexpr uint32_t kOffLo = pack4(-1, -kReg - 1, -kReg, -kReg + 1);
+  constexpr uint32_t kOffHi = pack4(1, kReg + 1, kReg, kReg - 1);
+  // 0xFF for the directions that leave through each side
+  constexpr uint32_t kWestLo = pack4(-1, -1, 0, 0), kWestHi = pack4(0, 0, 0, -1);
+  constexpr uint32_t kEastLo = pack4(0, 0, 0, -1), kEastHi = pack4(-1, -1, 0, 0);
+  constexpr uint32_t kNorthLo = pack4(0, -1, -1, -1), kNorthHi = 0u;
+  constexpr uint32_t kSouthLo = 0u, kSouthHi = pack4(0, -1, -1, -1);
+  const uint8_t *base = rcv + static_cast<size_t>(ry0) * W + rx0;
+  uint32_t v[kIters];
+#pragma unroll
+  for (int k = 0; k < kIters; k++) {
+    const int d = k * BLOCK + threadIdx.x;
+    const int row = d / kRowWords, col = d % kRowWords;
+    v[k] = d < kWords ? *reinterpret_cast<const uint32_t *>(base + static_cast<size_t>(row) * W +
+                                                            4 * col)
+                      : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < kIters; k++) {
+    const int d = k * BLOCK + threadIdx.x;
+    if (d >= kWords) continue;
+    const int row = d / kRowWords, col = d % kRowWords;
+    const uint32_t x = v[k];
+    // outlets are 0xFF bytes: m is 0xFF in their byte lanes
+    const uint32_t rb = (x >> 7) & 0x01010101u;
+    const uint32_t m = (rb << 8) - rb;
+    const uint32_t sel = (x & 0x07070707u & ~m) | (m & 0x0C0C0C0Cu);
+    uint32_t code = __builtin_amdgcn_perm(kOffHi, kOffLo, sel) | (rb << 7);
+    uint32_t ex = 0;
+    if (col == 0) ex |= __builtin_amdgcn_perm(kWestHi, kWestLo, sel) & 0x000000FFu;
+    if (col == kRowWords - 1) ex |= __builtin_amdgcn_perm(kEastHi, kEastLo, sel) & 0xFF000000u;
+    if (row == 0) ex |= __builtin_amdgcn_perm(kNorthHi, kNorthLo, sel);
+    if (row == kReg - 1) ex |= __builtin_amdgcn_perm(kSouthHi, kSouthLo, sel);
+    constexpr uint32_t kLeaves4 = 0x01010101u * static_cast<uint32_t>(kStopLeaves & 0xFF);
+    code = (code & ~ex) | (ex & kLeaves4);
+    const bool tile = static_cast<unsigned>(row - kHalo) < static_cast<unsigned>(kTile) &&
+                      static_cast<unsigned>(col - kHalo / 4) < static_cast<unsigned>(kTile / 4);
+    const uint32_t t = tile ? kAccTile : 0u;
+    uint4 q;
+    q.x = (code << 24) | t;
+    q.y = ((code << 16) & 0xFF000000u) | t;
+    q.z = ((code << 8) & 0xFF000000u) | t;
+    q.w = (code & 0xFF000000u) | t;
+    *reinterpret_cast<uint4 *>(&w_s[4 * d]) = q;
+  }
+}
+
 // Mark (bitmask over region cells) the halo cells on the receiver chains of
 // the tile's cells: walks start at tile cells whose receiver leaves the tile
 // and stop at an outlet, the region edge, a tile cell, an already marked cell
@@ -803,7 +869,7 @@ struct alignas(16) TilePlan {
 };
 
 struct AccShared {
-  uint32_t w_s[kRegCells];        // accumulation words
+  alignas(16) uint32_t w_s[kRegCells];   // accumulation words
   uint32_t mark[kRegCells / 32];
   uint16_t pre[kRegCells / 32];   // marks before each mark word
   uint32_t scan[1];               // number of marks
@@ -845,7 +911,10 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
   if (threadIdx.x == 0) sh.over = 0;
   if (threadIdx.x < kHalo + 1) sh.hist[threadIdx.x] = 0;
   if (threadIdx.x < kRegCells / 32) sh.mark[threadIdx.x] = 0;
-  load_region_words<BLOCK>(rcv, w_s, rx0, ry0, W, H);
+  if (rx0 >= 0 && ry0 >= 0 && rx0 + kReg <= W && ry0 + kReg <= H && W % 4 == 0)
+    load_region_words_x4<BLOCK>(rcv, w_s, rx0, ry0, W);
+  else
+    load_region_words<BLOCK>(rcv, w_s, rx0, ry0, W, H);
   __syncthreads();
   // Every region cell walks its receiver chain and adds 1 to each cell it
   // passes, itself included; the thread's chains advance together, one cell
