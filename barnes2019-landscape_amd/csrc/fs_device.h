@@ -1177,6 +1177,8 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
   int rank[kSlotPer];
 #pragma unroll
   for (int q = 0; q < kSlotPer; q++) {
+    rank[q] = -1;
+    if (q * kEroBlock + wave * kWave >= nslots) continue;   // the whole wave is past the slots
     const int sl = q * kEroBlock + threadIdx.x;
     const int d = sl < nslots ? d_s[sl] : 0xFF;
     const bool valid = d < kKeys;
@@ -1215,6 +1217,7 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kSlotPer; q++) {
+    if (q * kEroBlock + wave * kWave >= nslots) continue;
     const int sl = q * kEroBlock + threadIdx.x;
     if (rank[q] < 0) continue;
     const int d = d_s[sl];
