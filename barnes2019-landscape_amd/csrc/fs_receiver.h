@@ -96,11 +96,11 @@ __host__ __device__ inline int group_first(const double d[8], int first, double 
 __host__ __device__ inline int receiver_fast(double hc, const double nb[8], double dc, double dd) {
   double d[8];
   for (int n = 0; n < 8; n++) d[n] = hc - nb[n];
-  double Ac = -__builtin_huge_val(), Ad = -__builtin_huge_val();
-  for (int n = 0; n < 8; n += 2)
-    if (d[n] > Ac) Ac = d[n];
-  for (int n = 1; n < 8; n += 2)
-    if (d[n] > Ad) Ad = d[n];
+  // Group maxima.  fmax ignores NaN drops, as the reference's strict '>'
+  // does; it may return +0 where '>' keeps -0, but a group only counts when
+  // its maximum is positive.
+  const double Ac = fmax(fmax(d[0], d[2]), fmax(d[4], d[6]));
+  const double Ad = fmax(fmax(d[1], d[3]), fmax(d[5], d[7]));
   const bool pc = Ac > 0.0, pd = Ad > 0.0;
   if (!pc && !pd) return kNoReceiver;
   // quotients of drops in [2^-900, 2^900] by divisors in [2^-60, 2^60] are
