@@ -53,8 +53,8 @@ KERNEL_BYTES = {
 # that workload only.
 MEASURED_TRAFFIC_PER_CELL = {
     "k_tile_erode": 23.89,
-    "k_tile_acc": 8.60,
-    "k_receivers": 10.69,
+    "k_tile_acc": 8.61,
+    "k_receivers": 10.68,
 }
 
 
