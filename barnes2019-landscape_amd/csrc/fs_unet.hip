@@ -312,6 +312,156 @@ __global__ __launch_bounds__(256) void k_reduce_chunks(const float *__restrict__
   }
 }
 
+// ---------------------------------------------------------------------------
+// Weight gradient with all nine taps per workgroup:
+//   part[chunk][t][co][ci] = sum over the chunk's pixel tiles of
+//                            dz[p][co] * x[p + off_t][ci]
+// A workgroup owns CB outputs x 32 inputs and a run of 8 x 16 pixel tiles of
+// one image level; wave w handles the three taps of row dy = w - 1.  Per tile
+// the dz tile goes to LDS as [output][pixel] and three column-shifted copies
+// of the input patch (tile + one-pixel halo, zero padded) as
+// [input][patch row][16 columns], both transposed in registers in 8 x 8
+// blocks.  Every MFMA operand is then one aligned 16-byte LDS read: A = 8
+// consecutive pixels of one output, B = the same 8 pixels of one input moved
+// by the tap's offset.  Needs H % 8 == 0, W % 16 == 0, cout % CB == 0,
+// cin % 8 == 0.
+// ---------------------------------------------------------------------------
+constexpr int kDwTH = 8, kDwTW = 16, kDwTP = kDwTH * kDwTW;   // 128 pixels per tile
+constexpr int kDwAS = kDwTP + 8;            // A row stride in LDS (pixels + pad)
+constexpr int kDwBR = kDwTH + 2;            // patch rows
+constexpr int kDwBS = kDwBR * kDwTW + 8;    // B row stride per input channel
+constexpr int kDwThreads = 192;             // three waves: tap rows dy = -1, 0, 1
+
+// o[j] = element j of each of v[0..7]: an 8 x 8 transpose of 16-bit values.
+__device__ __forceinline__ void transpose8(const uint4 (&v)[8], uint4 (&o)[8]) {
+  const uint32_t *w[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) w[i] = reinterpret_cast<const uint32_t *>(&v[i]);
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    uint32_t *d = reinterpret_cast<uint32_t *>(&o[j]);
+    const uint32_t sel = (j & 1) ? 0x07060302u : 0x05040100u;   // high or low halves
+#pragma unroll
+    for (int k = 0; k < 4; k++) d[k] = __builtin_amdgcn_perm(w[2 * k + 1][j >> 1], w[2 * k][j >> 1], sel);
+  }
+}
+
+template <int CB>
+__global__ __launch_bounds__(kDwThreads) void k_conv3x3_dw_t(
+    const uint16_t *__restrict__ dz, int cout, const uint16_t *__restrict__ x, int cin,
+    float *__restrict__ part, int H, int W, int tiles_img, int tiles_x, long ntiles,
+    int tiles_per_chunk, int ci_blocks) {
+  constexpr int NA = CB / 16;
+  __shared__ __attribute__((aligned(16))) uint16_t As[CB * kDwAS];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[3 * 32 * kDwBS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int co0 = (blockIdx.x / ci_blocks) * CB, ci0 = (blockIdx.x % ci_blocks) * 32;
+  const long t0 = static_cast<long>(blockIdx.y) * tiles_per_chunk;
+  const long t1 = min(ntiles, t0 + tiles_per_chunk);
+  f32x4 acc[3][NA][2];
+#pragma unroll
+  for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+    for (int a = 0; a < NA; a++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) acc[dx][a][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (long tile = t0; tile < t1; tile++) {
+    const long b = tile / tiles_img;
+    const int ti = static_cast<int>(tile % tiles_img);
+    const int y0 = (ti / tiles_x) * kDwTH, x0 = (ti % tiles_x) * kDwTW;
+    // dz: blocks of 8 pixels (half a tile row) x 8 outputs
+    constexpr int kABlocks = (kDwTP / 8) * (CB / 8);
+    for (int blk = threadIdx.x; blk < kABlocks; blk += kDwThreads) {
+      const int pb = blk % (kDwTP / 8), cg = blk / (kDwTP / 8);
+      const int py = pb >> 1, px = (pb & 1) * 8;
+      const uint16_t *src = dz + ((b * H + y0 + py) * W + x0 + px) * cout + co0 + 8 * cg;
+      uint4 v[8], o[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) v[i] = *reinterpret_cast<const uint4 *>(src + static_cast<long>(i) * cout);
+      transpose8(v, o);
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        *reinterpret_cast<uint4 *>(&As[(8 * cg + j) * kDwAS + py * kDwTW + px]) = o[j];
+    }
+    // input patch: for each column shift dx, patch row, half row and group of
+    // 8 inputs, 8 pixels x 8 inputs
+    constexpr int kBBlocks = 3 * kDwBR * 2 * 4;
+    for (int blk = threadIdx.x; blk < kBBlocks; blk += kDwThreads) {
+      const int cg = blk & 3, h = (blk >> 2) & 1, pr = (blk >> 3) % kDwBR, dx = blk / (8 * kDwBR);
+      const int yy = y0 - 1 + pr, xs = x0 - 1 + dx + 8 * h;
+      const bool row_ok = yy >= 0 && yy < H && ci0 + 8 * cg < cin;
+      const uint16_t *src = x + ((b * H + (row_ok ? yy : 0)) * W) * cin + ci0 + 8 * cg;
+      uint4 v[8], o[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const int xx = xs + i;
+        v[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (row_ok && xx >= 0 && xx < W) v[i] = *reinterpret_cast<const uint4 *>(src + static_cast<long>(xx) * cin);
+      }
+      transpose8(v, o);
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        *reinterpret_cast<uint4 *>(&Bs[(dx * 32 + 8 * cg + j) * kDwBS + pr * kDwTW + 8 * h]) = o[j];
+    }
+    __syncthreads();
+    // k-steps of 32 pixels: tile rows 2s and 2s + 1
+#pragma unroll
+    for (int s = 0; s < kDwTH / 2; s++) {
+      uint4 av[NA];
+#pragma unroll
+      for (int a = 0; a < NA; a++)
+        av[a] = *reinterpret_cast<const uint4 *>(&As[(16 * a + r) * kDwAS + 32 * s + 8 * g]);
+      const int prow = 2 * s + (g >> 1) + wave;   // patch row of this lane's pixels
+#pragma unroll
+      for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          const uint4 bv = *reinterpret_cast<const uint4 *>(
+              &Bs[(dx * 32 + 16 * j + r) * kDwBS + prow * kDwTW + 8 * (g & 1)]);
+#pragma unroll
+          for (int a = 0; a < NA; a++) acc[dx][a][j] = mfma16(av[a], bv, acc[dx][a][j]);
+        }
+    }
+    __syncthreads();
+  }
+  float *dst = part + static_cast<long>(blockIdx.y) * 9 * cout * cin;
+#pragma unroll
+  for (int dx = 0; dx < 3; dx++) {
+    const int t = 3 * wave + dx;
+#pragma unroll
+    for (int a = 0; a < NA; a++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int co = co0 + 16 * a + 4 * g + i, ci = ci0 + 16 * j + r;
+          if (ci < cin) dst[(static_cast<long>(t) * cout + co) * cin + ci] = acc[dx][a][j][i];
+        }
+  }
+}
+
+// dw[i] = sum over chunks of part[chunk][i]; 64 elements x 16 chunk lanes per
+// workgroup, the chunk lanes combined in LDS.
+__global__ __launch_bounds__(1024) void k_reduce_chunks_wide(const float *__restrict__ part,
+                                                             float *__restrict__ dw, long n,
+                                                             int nchunks) {
+  __shared__ float red[16][64];
+  const int e = threadIdx.x & 63, cl = threadIdx.x >> 6;
+  const long i = static_cast<long>(blockIdx.x) * 64 + e;
+  float s = 0.f;
+  if (i < n)
+    for (int c = cl; c < nchunks; c += 16) s += part[static_cast<long>(c) * n + i];
+  red[cl][e] = s;
+  __syncthreads();
+  if (cl == 0 && i < n) {
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; k++) t += red[k][e];
+    dw[i] = t;
+  }
+}
+
 // db[c] += sum_p dz[p][c] over the block's pixel range; thread t sums the
 // 8-channel group t % C8 over every (256 / C8)-th pixel, C % 8 == 0, C <= 512
 __global__ __launch_bounds__(256) void k_bias_grad(const uint16_t *__restrict__ dz, int C,
@@ -704,6 +854,32 @@ int conv_fwd(fs_unet *u, const Conv &c, const Tensor &in, Tensor &out, int B) {
   return launch_conv(u, in.d, c.cin, c.wf, u->w + c.boff, out.d, c.cout, Hl, Wl, 1, B);
 }
 
+// Work split of the nine-tap weight-gradient kernel: about kDwTargetGroups
+// workgroups per layer, whole pixel tiles per chunk.  Levels whose size is
+// not a multiple of the 8 x 16 tile use k_conv3x3_dw.
+constexpr long kDwTargetGroups = 768;
+struct DwPlan {
+  bool tiled = false;
+  int cb = 0, ci_blocks = 0, blocks = 0, tiles_x = 0, tiles_img = 0, tpc = 0, nchunks = 0;
+  long ntiles = 0, max_chunks = 0;
+};
+DwPlan dw_plan(int cout, int cin, int Hl, int Wl, int B) {
+  DwPlan p;
+  p.tiled = Hl % kDwTH == 0 && Wl % kDwTW == 0 && cin % 8 == 0 && cout % 32 == 0;
+  if (!p.tiled) return p;
+  p.cb = cout % 64 == 0 ? 64 : 32;
+  p.ci_blocks = (cin + 31) / 32;
+  p.blocks = cout / p.cb * p.ci_blocks;
+  p.tiles_x = Wl / kDwTW;
+  p.tiles_img = (Hl / kDwTH) * p.tiles_x;
+  p.ntiles = static_cast<long>(B) * p.tiles_img;
+  const long want = (kDwTargetGroups + p.blocks - 1) / p.blocks;
+  p.max_chunks = std::min(want, p.ntiles);
+  p.tpc = static_cast<int>((p.ntiles + p.max_chunks - 1) / p.max_chunks);
+  p.nchunks = static_cast<int>((p.ntiles + p.tpc - 1) / p.tpc);
+  return p;
+}
+
 // Gradients of one conv+ReLU layer from gout (gradient of its output):
 // weight and bias gradients, and the input gradient into gin (if given).
 int conv_bwd(fs_unet *u, const Conv &c, const Tensor &in, const Tensor &out, const Tensor &gout,
@@ -714,17 +890,35 @@ int conv_bwd(fs_unet *u, const Conv &c, const Tensor &in, const Tensor &out, con
                      gout.d, out.d, u->gz.d, M * c.cout / 8);
   FS_TRY(check_launch_u());
   // weight gradient, split over pixel chunks
-  const long chunk = std::max<long>(2048, (M + 63) / 64 / 32 * 32);
-  const int nchunks = static_cast<int>((M + chunk - 1) / chunk);
   const size_t wn = 9ul * c.cout * c.cin;
-  if (static_cast<size_t>(nchunks) * wn > u->part_n) return FS_ERR_INTERNAL;
-  const int tiles = ((c.cout + 63) / 64) * ((c.cin + 63) / 64);
-  hipLaunchKernelGGL(k_conv3x3_dw, dim3(tiles, 9, nchunks), dim3(256), 0, u->stream, u->gz.d,
-                     c.cout, in.d, c.cin, u->part, Hl, Wl, chunk, M);
-  FS_TRY(check_launch_u());
-  hipLaunchKernelGGL(k_reduce_chunks, dim3(grid_for(static_cast<long>(wn))), dim3(256), 0,
-                     u->stream, u->part, u->g + c.woff, static_cast<long>(wn), nchunks);
-  FS_TRY(check_launch_u());
+  const DwPlan dp = dw_plan(c.cout, c.cin, Hl, Wl, B);
+  if (dp.tiled) {
+    if (static_cast<size_t>(dp.nchunks) * wn > u->part_n) return FS_ERR_INTERNAL;
+    if (dp.cb == 64)
+      hipLaunchKernelGGL(k_conv3x3_dw_t<64>, dim3(dp.blocks, dp.nchunks), dim3(kDwThreads), 0,
+                         u->stream, u->gz.d, c.cout, in.d, c.cin, u->part, Hl, Wl, dp.tiles_img,
+                         dp.tiles_x, dp.ntiles, dp.tpc, dp.ci_blocks);
+    else
+      hipLaunchKernelGGL(k_conv3x3_dw_t<32>, dim3(dp.blocks, dp.nchunks), dim3(kDwThreads), 0,
+                         u->stream, u->gz.d, c.cout, in.d, c.cin, u->part, Hl, Wl, dp.tiles_img,
+                         dp.tiles_x, dp.ntiles, dp.tpc, dp.ci_blocks);
+    FS_TRY(check_launch_u());
+    hipLaunchKernelGGL(k_reduce_chunks_wide, dim3(static_cast<unsigned>((wn + 63) / 64)),
+                       dim3(1024), 0, u->stream, u->part, u->g + c.woff, static_cast<long>(wn),
+                       dp.nchunks);
+    FS_TRY(check_launch_u());
+  } else {
+    const long chunk = std::max<long>(2048, (M + 63) / 64 / 32 * 32);
+    const int nchunks = static_cast<int>((M + chunk - 1) / chunk);
+    if (static_cast<size_t>(nchunks) * wn > u->part_n) return FS_ERR_INTERNAL;
+    const int tiles = ((c.cout + 63) / 64) * ((c.cin + 63) / 64);
+    hipLaunchKernelGGL(k_conv3x3_dw, dim3(tiles, 9, nchunks), dim3(256), 0, u->stream, u->gz.d,
+                       c.cout, in.d, c.cin, u->part, Hl, Wl, chunk, M);
+    FS_TRY(check_launch_u());
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(grid_for(static_cast<long>(wn))), dim3(256), 0,
+                       u->stream, u->part, u->g + c.woff, static_cast<long>(wn), nchunks);
+    FS_TRY(check_launch_u());
+  }
   const long bchunk = 4096;
   hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(256), 0, u->stream, u->g + c.boff,
                      static_cast<long>(c.cout), 0.f);
@@ -944,6 +1138,8 @@ int fs_unet_create(int height, int width, int max_batch, uint64_t seed, int devi
     const long M = pixels(u, c.level, max_batch);
     const long chunk = std::max<long>(2048, (M + 63) / 64 / 32 * 32);
     part = std::max(part, static_cast<size_t>((M + chunk - 1) / chunk) * 9 * c.cout * c.cin);
+    const DwPlan dp = dw_plan(c.cout, c.cin, height >> c.level, width >> c.level, max_batch);
+    if (dp.tiled) part = std::max(part, static_cast<size_t>(dp.max_chunks) * 9 * c.cout * c.cin);
   }
   u->part_n = part;
   FS_TRY(fs::dev_alloc(&u->part, part));

@@ -62,8 +62,12 @@ def test_forward_matches_reference(H, W, B):
     assert np.corrcoef(y.ravel(), ref.ravel())[0, 1] > 0.999
 
 
-def test_gradients_match_reference():
-    H, W, B = 32, 32, 2
+# The nine-tap weight-gradient kernel needs level sizes that are multiples of
+# 8 rows x 16 columns; other levels use the per-tap kernel.  32x32 and 64x96
+# run the nine-tap kernel at levels 0-1, 96x64 at levels 0-2.
+@pytest.mark.parametrize("H,W,B", [(32, 32, 2), (64, 96, 2), (96, 64, 1)])
+def test_gradients_match_reference(H, W, B):
+    note("unet gradients %dx%d batch %d" % (H, W, B))
     terrain, params, target = sample(5, B, H, W)
     with fa.UNet(H, W, max_batch=B, seed=21) as net:
         p = perturbed_params(net, 22)
