@@ -323,8 +323,9 @@ __global__ __launch_bounds__(256) void k_reduce_chunks(const float *__restrict__
 // [input][patch row][16 columns], both transposed in registers in 8 x 8
 // blocks.  Every MFMA operand is then one aligned 16-byte LDS read: A = 8
 // consecutive pixels of one output, B = the same 8 pixels of one input moved
-// by the tap's offset.  Needs H % 8 == 0, W % 16 == 0, cout % CB == 0,
-// cin % 8 == 0.
+// by the tap's offset.  With db, the workgroups of the first input block
+// also add the bias gradient sum_p dz[p][co] of their pixels to db.  Needs
+// H % 8 == 0, W % 16 == 0, cout % CB == 0, cin % 8 == 0.
 // ---------------------------------------------------------------------------
 constexpr int kDwTH = 8, kDwTW = 16, kDwTP = kDwTH * kDwTW;   // 128 pixels per tile
 constexpr int kDwAS = kDwTP + 8;            // A row stride in LDS (pixels + pad)
@@ -349,9 +350,11 @@ __device__ __forceinline__ void transpose8(const uint4 (&v)[8], uint4 (&o)[8]) {
 template <int CB>
 __global__ __launch_bounds__(kDwThreads) void k_conv3x3_dw_t(
     const uint16_t *__restrict__ dz, int cout, const uint16_t *__restrict__ x, int cin,
-    float *__restrict__ part, int H, int W, int tiles_img, int tiles_x, long ntiles,
-    int tiles_per_chunk, int ci_blocks) {
+    float *__restrict__ part, float *__restrict__ db, int H, int W, int tiles_img, int tiles_x,
+    long ntiles, int tiles_per_chunk, int ci_blocks) {
   constexpr int NA = CB / 16;
+  constexpr int kABlocks = (kDwTP / 8) * (CB / 8);
+  static_assert(kABlocks <= kDwThreads, "one dz block per thread");
   __shared__ __attribute__((aligned(16))) uint16_t As[CB * kDwAS];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[3 * 32 * kDwBS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -366,19 +369,32 @@ __global__ __launch_bounds__(kDwThreads) void k_conv3x3_dw_t(
     for (int a = 0; a < NA; a++)
 #pragma unroll
       for (int j = 0; j < 2; j++) acc[dx][a][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bool bias = db != nullptr && blockIdx.x % ci_blocks == 0;
+  float bsum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};   // outputs of this thread's dz block
   for (long tile = t0; tile < t1; tile++) {
     const long b = tile / tiles_img;
     const int ti = static_cast<int>(tile % tiles_img);
     const int y0 = (ti / tiles_x) * kDwTH, x0 = (ti % tiles_x) * kDwTW;
-    // dz: blocks of 8 pixels (half a tile row) x 8 outputs
-    constexpr int kABlocks = (kDwTP / 8) * (CB / 8);
-    for (int blk = threadIdx.x; blk < kABlocks; blk += kDwThreads) {
+    // dz: blocks of 8 pixels (half a tile row) x 8 outputs, one per thread
+    if (static_cast<int>(threadIdx.x) < kABlocks) {
+      const int blk = threadIdx.x;
       const int pb = blk % (kDwTP / 8), cg = blk / (kDwTP / 8);
       const int py = pb >> 1, px = (pb & 1) * 8;
       const uint16_t *src = dz + ((b * H + y0 + py) * W + x0 + px) * cout + co0 + 8 * cg;
       uint4 v[8], o[8];
 #pragma unroll
       for (int i = 0; i < 8; i++) v[i] = *reinterpret_cast<const uint4 *>(src + static_cast<long>(i) * cout);
+      if (bias) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          const uint32_t *w = reinterpret_cast<const uint32_t *>(&v[i]);
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            bsum[2 * k] += __uint_as_float(w[k] << 16);
+            bsum[2 * k + 1] += __uint_as_float(w[k] & 0xFFFF0000u);
+          }
+        }
+      }
       transpose8(v, o);
 #pragma unroll
       for (int j = 0; j < 8; j++)
@@ -424,6 +440,22 @@ __global__ __launch_bounds__(kDwThreads) void k_conv3x3_dw_t(
         }
     }
     __syncthreads();
+  }
+  if (bias) {
+    // the 16 pixel blocks of each output group, summed in LDS (Bs is free:
+    // the last tile ended with a barrier)
+    float *red = reinterpret_cast<float *>(Bs);
+    if (static_cast<int>(threadIdx.x) < kABlocks) {
+      const int pb = threadIdx.x % (kDwTP / 8), cg = threadIdx.x / (kDwTP / 8);
+#pragma unroll
+      for (int e = 0; e < 8; e++) red[pb * CB + 8 * cg + e] = bsum[e];
+    }
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < CB) {
+      float sum = 0.f;
+      for (int pb = 0; pb < kDwTP / 8; pb++) sum += red[pb * CB + threadIdx.x];
+      atomicAdd(&db[co0 + threadIdx.x], sum);
+    }
   }
   float *dst = part + static_cast<long>(blockIdx.y) * 9 * cout * cin;
 #pragma unroll
@@ -894,14 +926,17 @@ int conv_bwd(fs_unet *u, const Conv &c, const Tensor &in, const Tensor &out, con
   const DwPlan dp = dw_plan(c.cout, c.cin, Hl, Wl, B);
   if (dp.tiled) {
     if (static_cast<size_t>(dp.nchunks) * wn > u->part_n) return FS_ERR_INTERNAL;
+    // the bias gradient is summed by the same kernel
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(256), 0, u->stream, u->g + c.boff,
+                       static_cast<long>(c.cout), 0.f);
     if (dp.cb == 64)
       hipLaunchKernelGGL(k_conv3x3_dw_t<64>, dim3(dp.blocks, dp.nchunks), dim3(kDwThreads), 0,
-                         u->stream, u->gz.d, c.cout, in.d, c.cin, u->part, Hl, Wl, dp.tiles_img,
-                         dp.tiles_x, dp.ntiles, dp.tpc, dp.ci_blocks);
+                         u->stream, u->gz.d, c.cout, in.d, c.cin, u->part, u->g + c.boff, Hl, Wl,
+                         dp.tiles_img, dp.tiles_x, dp.ntiles, dp.tpc, dp.ci_blocks);
     else
       hipLaunchKernelGGL(k_conv3x3_dw_t<32>, dim3(dp.blocks, dp.nchunks), dim3(kDwThreads), 0,
-                         u->stream, u->gz.d, c.cout, in.d, c.cin, u->part, Hl, Wl, dp.tiles_img,
-                         dp.tiles_x, dp.ntiles, dp.tpc, dp.ci_blocks);
+                         u->stream, u->gz.d, c.cout, in.d, c.cin, u->part, u->g + c.boff, Hl, Wl,
+                         dp.tiles_img, dp.tiles_x, dp.ntiles, dp.tpc, dp.ci_blocks);
     FS_TRY(check_launch_u());
     hipLaunchKernelGGL(k_reduce_chunks_wide, dim3(static_cast<unsigned>((wn + 63) / 64)),
                        dim3(1024), 0, u->stream, u->part, u->g + c.woff, static_cast<long>(wn),
@@ -919,12 +954,14 @@ int conv_bwd(fs_unet *u, const Conv &c, const Tensor &in, const Tensor &out, con
                        u->stream, u->part, u->g + c.woff, static_cast<long>(wn), nchunks);
     FS_TRY(check_launch_u());
   }
-  const long bchunk = 4096;
-  hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(256), 0, u->stream, u->g + c.boff,
-                     static_cast<long>(c.cout), 0.f);
-  hipLaunchKernelGGL(k_bias_grad, dim3((M + bchunk - 1) / bchunk), dim3(256), 0, u->stream,
-                     u->gz.d, c.cout, u->g + c.boff, M, bchunk);
-  FS_TRY(check_launch_u());
+  if (!dp.tiled) {
+    const long bchunk = 4096;
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(256), 0, u->stream, u->g + c.boff,
+                       static_cast<long>(c.cout), 0.f);
+    hipLaunchKernelGGL(k_bias_grad, dim3((M + bchunk - 1) / bchunk), dim3(256), 0, u->stream,
+                       u->gz.d, c.cout, u->g + c.boff, M, bchunk);
+    FS_TRY(check_launch_u());
+  }
   if (gin) FS_TRY(launch_conv(u, u->gz.d, c.cout, c.wb, nullptr, gin->d, c.cin, Hl, Wl, 0, B));
   return FS_OK;
 }
