@@ -40,8 +40,10 @@ KERNEL_BYTES = {
     "k_bfs": 9,            # stack entry 4 + donor mask 1 + donor index out 4
     "k_flowacc": 13,       # stack 4 + mask 1 + donor counts 4 (one donor/cell) + count 4
     "k_erode": 33,         # stack 4 + rcv 1 + count 4 + h 8 + receiver h 8 + h out 8
-    # tile path, per cell, at the survey's per-stage figures (79 B/cell-step in
-    # total): donors 2 + order 10 + accumulation 21; uplift + erosion 37
+    # tile path, per cell, at the per-stage algorithmic figures of a whole
+    # time-step (79 B/cell-step in total: receivers 9, donors 2, order 10,
+    # accumulation 21, uplift + erosion 37): donors + order + accumulation 33;
+    # uplift + erosion 37
     "k_tile_acc": 33,
     "k_tile_erode": 37,
 }
@@ -306,8 +308,9 @@ def bench_unet(args, fa):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(independent realizations shard by whole grids, see DESIGN.md)" % args.gpus,
+        print("bench.py: the benchmark runs on one GPU; --gpus %d is not supported "
+              "(a multi-GPU run would shard ensembles by whole realizations and a single "
+              "grid by row strips with a 16-row halo)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     import fastscape_amd as fa
