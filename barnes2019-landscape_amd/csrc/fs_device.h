@@ -1247,6 +1247,12 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
       const int rs = rs_s[sl];
       const int nb = rs & kPlanSlot;
       if (nb < nslots) {                      // always, for a cell above level 0
+        // the cell's own inputs do not depend on its receiver: read them
+        // before the acquire wait, which later loads may not pass
+        const uint32_t a = c_s[sl];
+        const int ti = 2 * static_cast<int>(a) + (rs >> 15);
+        const double fact = a < kEroFactCache ? ft_s[ti] : ftab[ti];
+        const double h0 = hs[sl] + k.ud;
         int spins = 0;
         while (__hip_atomic_load(&d_s[nb], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) !=
                kDone) {
@@ -1256,10 +1262,7 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
           }
           __builtin_amdgcn_s_sleep(1);
         }
-        const uint32_t a = c_s[sl];
-        const int ti = 2 * static_cast<int>(a) + (rs >> 15);
-        const double fact = a < kEroFactCache ? ft_s[ti] : ftab[ti];
-        hs[sl] = newton_update<NMODE>(hs[sl] + k.ud, hs[nb], fact, k, hdr);
+        hs[sl] = newton_update<NMODE>(h0, hs[nb], fact, k, hdr);
       } else {
         atomicOr(&hdr[2], kErrIndex);
       }
