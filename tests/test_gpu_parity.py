@@ -82,9 +82,11 @@ def test_stages_vs_reference_golden(oracle, fx):
             h_in = g["h_%d" % s]
 
 
+# 4096x4096 is the single-grid workload whose receivers and levels are
+# checked bit for bit every step (about 25 s, mostly the oracle's steps).
 @pytest.mark.parametrize("W,H,preset,steps", [
     (64, 64, 1, 4), (257, 193, 1, 4), (193, 257, 0, 4), (1000, 1000, 1, 3), (1000, 1000, 0, 2),
-    (5, 5, 1, 2), (6, 300, 0, 2), (300, 6, 1, 2)])
+    (5, 5, 1, 2), (6, 300, 0, 2), (300, 6, 1, 2), (4096, 4096, 1, 2)])
 def test_step_sync_vs_oracle(oracle, W, H, preset, steps):
     note("step-sync %dx%d preset %d" % (W, H, preset))
     op = oracle_params(oracle, preset)
