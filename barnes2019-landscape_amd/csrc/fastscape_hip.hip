@@ -46,6 +46,15 @@
 
 #include "fs_device.h"
 
+// The same tile kernels with a 32-cell halo, for drainage 17..32 levels deep.
+namespace {
+namespace tile32 {
+#define FS_TILE_HALO 32
+#include "fs_tile.h"
+#undef FS_TILE_HALO
+}  // namespace tile32
+}  // namespace
+
 // ===========================================================================
 // host side
 // ===========================================================================
@@ -73,6 +82,7 @@ struct fs_model {
   uint32_t *cnt = nullptr;
   double *ftab = nullptr;   // erosion factor per (upstream count, diagonal)
   TilePlan *plan = nullptr; // per-tile erosion plan of the tile path
+  tile32::TilePlan *plan32 = nullptr;  // the same for the 32-cell halo (on first use)
   uint64_t *status = nullptr;
   unsigned long long *ticket = nullptr;
 
@@ -84,7 +94,10 @@ struct fs_model {
   bool have_step = false;
   bool order_current = false;      // stack and donor masks match rcv
   int path = FS_PATH_AUTO;
-  uint64_t tile_steps = 0, level_steps = 0;
+  uint64_t tile_steps = 0, level_steps = 0, tile32_steps = 0;
+  // halo the next step tries first: 16 or 32; 0 = level path (the last
+  // step's drainage was deeper than 32 levels)
+  int tile_halo = kHalo;
   unsigned long long ticket_base = 0;
   uint32_t epoch = 0;
 
@@ -184,7 +197,8 @@ void free_model(fs_model *m) {
                     static_cast<void *>(m->stack), static_cast<void *>(m->hdr),
                     static_cast<void *>(m->levels_d), static_cast<void *>(m->cnt),
                     static_cast<void *>(m->ftab), static_cast<void *>(m->status),
-                    static_cast<void *>(m->ticket), static_cast<void *>(m->plan)})
+                    static_cast<void *>(m->ticket), static_cast<void *>(m->plan),
+                    static_cast<void *>(m->plan32)})
     if (ptr) (void)hipFree(ptr);
   if (m->hdr_h) (void)hipHostFree(m->hdr_h);
   if (m->levels_h) (void)hipHostFree(m->levels_h);
@@ -517,44 +531,83 @@ int level_path(fs_model *m, hipEvent_t *ev) {
 }
 
 template <int NMODE>
-int launch_tile_erode(fs_model *m, int tiles, int tiles_x) {
-  hipLaunchKernelGGL(k_tile_erode<NMODE>, dim3(tiles), dim3(kEroBlock), 0, m->stream, m->h, m->h_alt,
-                     m->plan, m->lvl, m->cnt, m->ftab, m->W, m->H, tiles_x, tiles, m->ek, m->hdr);
+int launch_tile_erode(fs_model *m, int tiles, int tiles_x, int halo) {
+  if (halo == kHalo)
+    hipLaunchKernelGGL(k_tile_erode<NMODE>, dim3(tiles), dim3(kEroBlock), 0, m->stream, m->h,
+                       m->h_alt, m->plan, m->lvl, m->cnt, m->ftab, m->W, m->H, tiles_x, tiles,
+                       m->ek, m->hdr);
+  else
+    hipLaunchKernelGGL(tile32::k_tile_erode<NMODE>, dim3(tiles), dim3(tile32::kEroBlock), 0,
+                       m->stream, m->h, m->h_alt, m->plan32, m->lvl, m->cnt, m->ftab, m->W, m->H,
+                       tiles_x, tiles, m->ek, m->hdr);
   return check_launch();
 }
 
-// The tile path (every level <= kHalo): two region kernels and one header
-// read.  Returns FS_OK with *done = false when the levels are too deep; the
-// caller then runs the level path on the same receivers.
+// Accumulation with the given halo (16 or 32) and the header read after it.
+// Returns FS_OK with *fits = false when some level exceeds the halo (or a
+// tile's chains need more halo cells than its plan holds); hdr_h[0] is then
+// the deepest level among the cells that reach an outlet within the region.
+int tile_accumulate(fs_model *m, int halo, int tiles, int tiles_x, hipEvent_t ev_done,
+                    bool *fits) {
+  hipStream_t s = m->stream;
+  *fits = false;
+  hipLaunchKernelGGL(k_fill_counts, dim3(1), dim3(64), 0, s, m->hdr, 0);
+  FS_TRY(check_launch());
+  FS_TRY(kt_begin(m, FS_K_TILE_ACC));
+  if (halo == kHalo) {
+    hipLaunchKernelGGL(k_tile_acc, dim3(tiles), dim3(kAccBlock), 0, s, m->rcv, m->cnt, m->lvl,
+                       m->plan, m->hdr, m->W, m->H, tiles_x, tiles);
+  } else {
+    if (!m->plan32) FS_TRY(fs::dev_alloc(&m->plan32, static_cast<size_t>(tiles)));
+    hipLaunchKernelGGL(tile32::k_tile_acc, dim3(tiles), dim3(tile32::kAccBlock), 0, s, m->rcv,
+                       m->cnt, m->lvl, m->plan32, m->hdr, m->W, m->H, tiles_x, tiles);
+  }
+  FS_TRY(check_launch());
+  FS_TRY(kt_end(m));
+  FS_HIP_TRY(hipEventRecord(ev_done, s));
+  FS_TRY(fetch_header(m, 0));
+  FS_TRY(device_error_status(m->hdr_h[2]));
+  *fits = m->hdr_h[3] == 0;
+  return FS_OK;
+}
+
+// The tile path (every level <= 32): accumulation with a 16-cell halo, or a
+// 32-cell one when the drainage is deeper, then erosion with the same halo,
+// one header read in between.  Returns FS_OK with *done = false when the
+// levels are too deep for either; the caller then runs the level path on the
+// same receivers.  The halo that fitted is tried first at the next step.
 int tile_path(fs_model *m, hipEvent_t *ev, bool *done) {
   *done = false;
   hipStream_t s = m->stream;
   const int tiles_x = (m->W + kTile - 1) / kTile, tiles_y = (m->H + kTile - 1) / kTile;
   const int tiles = tiles_x * tiles_y;
-  hipLaunchKernelGGL(k_fill_counts, dim3(1), dim3(64), 0, s, m->hdr, 0);
-  FS_TRY(check_launch());
-  FS_TRY(kt_begin(m, FS_K_TILE_ACC));
-  hipLaunchKernelGGL(k_tile_acc, dim3(tiles), dim3(kAccBlock), 0, s, m->rcv, m->cnt, m->lvl, m->plan,
-                     m->hdr, m->W, m->H, tiles_x, tiles);
-  FS_TRY(check_launch());
-  FS_TRY(kt_end(m));
-  FS_HIP_TRY(hipEventRecord(ev[2], s));
-  FS_TRY(fetch_header(m, 0));
+  int halo = m->tile_halo == tile32::kHalo ? tile32::kHalo : kHalo;
+  bool fits = false;
+  FS_TRY(tile_accumulate(m, halo, tiles, tiles_x, ev[2], &fits));
   FS_TRY(stage_collect(m, m->ev_cur ^ 1));
   FS_TRY(kt_collect(m));
-  FS_TRY(device_error_status(m->hdr_h[2]));
-  if (m->hdr_h[3] != 0) return FS_OK;       // some level exceeds the halo
+  if (!fits && halo == kHalo) {
+    halo = tile32::kHalo;
+    FS_TRY(tile_accumulate(m, halo, tiles, tiles_x, ev[2], &fits));
+  }
+  if (!fits) {
+    m->tile_halo = kHalo;                   // the level path sets it from the depth it finds
+    return FS_OK;
+  }
+  // a 32-halo step whose levels all fit 16 tries the 16-cell halo next
+  m->tile_halo = (halo == tile32::kHalo && m->hdr_h[0] <= kHalo) ? kHalo : halo;
   m->order_current = false;                 // levels, stack, donor masks on request
   FS_HIP_TRY(hipEventRecord(ev[3], s));
   FS_HIP_TRY(hipEventRecord(ev[4], s));
   FS_HIP_TRY(hipEventRecord(ev[5], s));
   FS_TRY(kt_begin(m, FS_K_TILE_ERODE));
-  if (m->nmode == 1) FS_TRY(launch_tile_erode<1>(m, tiles, tiles_x));
-  else if (m->nmode == 2) FS_TRY(launch_tile_erode<2>(m, tiles, tiles_x));
-  else FS_TRY(launch_tile_erode<0>(m, tiles, tiles_x));
+  if (m->nmode == 1) FS_TRY(launch_tile_erode<1>(m, tiles, tiles_x, halo));
+  else if (m->nmode == 2) FS_TRY(launch_tile_erode<2>(m, tiles, tiles_x, halo));
+  else FS_TRY(launch_tile_erode<0>(m, tiles, tiles_x, halo));
   FS_TRY(kt_end(m));
   std::swap(m->h, m->h_alt);
   m->tile_steps++;
+  if (halo != kHalo) m->tile32_steps++;
   *done = true;
   return FS_OK;
 }
@@ -566,8 +619,13 @@ int step_once(fs_model *m) {
   FS_TRY(launch_receivers(m));
   FS_HIP_TRY(hipEventRecord(ev[1], m->stream));
   bool done = false;
-  if (m->path == FS_PATH_AUTO) FS_TRY(tile_path(m, ev, &done));
-  if (!done) FS_TRY(level_path(m, ev));
+  if (m->path == FS_PATH_AUTO && m->tile_halo != 0) FS_TRY(tile_path(m, ev, &done));
+  if (!done) {
+    FS_TRY(level_path(m, ev));
+    // the next step tries the tile halo this step's depth would have fitted
+    const int D = static_cast<int>(m->counts.size()) - 1;
+    m->tile_halo = D <= kHalo ? kHalo : D <= tile32::kHalo ? tile32::kHalo : 0;
+  }
   FS_HIP_TRY(hipEventRecord(ev[6], m->stream));
   m->ev_pending[m->ev_cur] = true;
   m->ev_cur ^= 1;
@@ -856,6 +914,13 @@ int fs_path_stats(fs_model *m, uint64_t *tile_steps, uint64_t *level_steps) {
   if (!m) return FS_ERR_INVALID_ARGUMENT;
   if (tile_steps) *tile_steps = m->tile_steps;
   if (level_steps) *level_steps = m->level_steps;
+  return FS_OK;
+}
+
+int fs_tile_stats(fs_model *m, uint64_t *halo16_steps, uint64_t *halo32_steps) {
+  if (!m) return FS_ERR_INVALID_ARGUMENT;
+  if (halo16_steps) *halo16_steps = m->tile_steps - m->tile32_steps;
+  if (halo32_steps) *halo32_steps = m->tile32_steps;
   return FS_OK;
 }
 

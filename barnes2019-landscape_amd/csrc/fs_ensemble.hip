@@ -426,7 +426,7 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_step(EnsArgs a) {
       const int t = item - a.tabA_off[e];
       const int over = tile_acc_body<kEnsBlock>(
           sh.acc, a.rcv + g.off, a.cnt + g.off, a.lvl + g.off, a.plan + g.plan_off + t, g.W, g.H,
-          t, g.tiles_x, a.hist + static_cast<size_t>(r) * kHistLen);
+          t, g.tiles_x, a.hist + static_cast<size_t>(r) * kHistLen, nullptr);
       if (threadIdx.x == 0 && over) atomicAdd(&a.rh[static_cast<size_t>(r) * kRH + kRhOver], 1);
       __syncthreads();
     }

@@ -26,13 +26,14 @@ KERNELS = ["k_receivers", "k_topology", "k_roots", "k_bfs", "k_flowacc", "k_upli
 PATH_AUTO = 0
 PATH_LEVELS = 1
 TILE_HALO = 16
+TILE_HALO_DEEP = 32
 
 # Every symbol declared in include/fastscape.h.
 EXPORTED = [
     "fs_params_preset", "fs_create", "fs_create_from_h", "fs_destroy", "fs_run", "fs_step",
     "fs_get_h", "fs_set_h", "fs_get_timers", "fs_reset_timers", "fs_get_receivers",
     "fs_get_donors", "fs_get_stack", "fs_get_levels", "fs_get_accum", "fs_device_h",
-    "fs_stream", "fs_synchronize", "fs_set_step_path", "fs_path_stats", "fs_set_kernel_timing", "fs_kernel_stats",
+    "fs_stream", "fs_synchronize", "fs_set_step_path", "fs_path_stats", "fs_tile_stats", "fs_set_kernel_timing", "fs_kernel_stats",
     "fs_kernel_name", "fs_calibrate_copy", "fs_ensemble_create", "fs_ensemble_destroy",
     "fs_ensemble_size", "fs_ensemble_run", "fs_ensemble_step", "fs_ensemble_get_h",
     "fs_ensemble_set_h", "fs_ensemble_get_receivers", "fs_ensemble_get_accum",
@@ -118,6 +119,7 @@ def _lib():
         "fs_synchronize": (ctypes.c_int, [_vp]),
         "fs_set_step_path": (ctypes.c_int, [_vp, ctypes.c_int]),
         "fs_path_stats": (ctypes.c_int, [_vp, _u64p, _u64p]),
+        "fs_tile_stats": (ctypes.c_int, [_vp, _u64p, _u64p]),
         "fs_set_kernel_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
         "fs_kernel_stats": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _u64p]),
         "fs_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
@@ -347,6 +349,13 @@ class FastScape:
         lv = ctypes.c_uint64(0)
         _check(_lib().fs_path_stats(self._m, ctypes.byref(t), ctypes.byref(lv)), "fs_path_stats")
         return {"tile": t.value, "levels": lv.value}
+
+    def tile_stats(self):
+        """Tile-path steps split by halo: {"halo16": n, "halo32": n}."""
+        t16 = ctypes.c_uint64(0)
+        t32 = ctypes.c_uint64(0)
+        _check(_lib().fs_tile_stats(self._m, ctypes.byref(t16), ctypes.byref(t32)), "fs_tile_stats")
+        return {"halo16": t16.value, "halo32": t32.value}
 
     def set_kernel_timing(self, enable):
         _check(_lib().fs_set_kernel_timing(self._m, 1 if enable else 0), "fs_set_kernel_timing")

@@ -135,16 +135,25 @@ int fs_reset_timers(fs_model *m);
 /* ---- step path ------------------------------------------------------------
  * FS_PATH_AUTO (default): when every cell is at most FS_TILE_HALO receiver
  * hops from its outlet, accumulation and erosion run per 64x64 tile with a
- * 16-cell halo in LDS (two kernels per step); otherwise the step falls back
- * to the level path.  FS_PATH_LEVELS: always the level path -- one launch
- * per level for order, accumulation and erosion, as the reference's stages
- * (RB:366-372).  Both give bit-identical results.  No reference counterpart. */
+ * 16-cell halo in LDS (two kernels per step); when every cell is at most
+ * FS_TILE_HALO_DEEP hops from it, the same two kernels run with a 32-cell
+ * halo; otherwise the step falls back to the level path.  A tile step whose
+ * chains need more halo cells than a tile's plan holds (256 with the 16-cell
+ * halo, 512 with the 32-cell one) also falls back.  FS_PATH_LEVELS: always
+ * the level path -- one launch per level for order, accumulation and
+ * erosion, as the reference's stages (RB:366-372).  All give bit-identical
+ * results.  No reference counterpart. */
 #define FS_PATH_AUTO   0
 #define FS_PATH_LEVELS 1
 #define FS_TILE_HALO   16
+#define FS_TILE_HALO_DEEP 32
 int fs_set_step_path(fs_model *m, int path);
-/* Number of time-steps taken by each path since creation. */
+/* Number of time-steps taken by each path since creation; tile_steps counts
+ * both halos. */
 int fs_path_stats(fs_model *m, uint64_t *tile_steps, uint64_t *level_steps);
+/* Tile-path steps split by halo (16-cell, 32-cell).  No reference
+ * counterpart. */
+int fs_tile_stats(fs_model *m, uint64_t *halo16_steps, uint64_t *halo32_steps);
 
 /* ---- state of the last completed time-step (parity checks) -------------- */
 /* After a tile-path step, levels[], the stack and the donor lists are

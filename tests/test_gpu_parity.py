@@ -191,6 +191,46 @@ def test_deep_drainage_vs_oracle(oracle, kind):
                 hh = st["h"]
 
 
+def cone_terrain(W, H, period, offset):
+    """Conical basins `period` cells wide: each cell drains to its basin's
+    centre, up to period/2 receiver hops away."""
+    x = np.arange(W, dtype=np.float64)[None, :]
+    y = np.arange(H, dtype=np.float64)[:, None]
+    cx = (x + offset) % period - period / 2 + 0.5
+    cy = (y + offset) % period - period / 2 + 0.5
+    h = 1.0 + np.sqrt(cx * cx + cy * cy) + 1e-7 * ((x * 7 + y * 13) % 11)
+    h[:2, :] = h[-2:, :] = 0
+    h[:, :2] = h[:, -2:] = 0
+    return np.ascontiguousarray(h).ravel()
+
+
+# 48-cell basins: 25 levels and at most 460 halo cells on any tile's chains,
+# within the 32-cell halo's plan; 60-cell basins offset by 3: 31 levels but
+# up to 1642 halo cells on one tile's chains, beyond it (level path).
+@pytest.mark.parametrize("period,offset,halo32", [(48, 0, True), (60, 3, False)])
+def test_deep_tile_path_vs_oracle(oracle, period, offset, halo32):
+    W, H = 200, 160
+    h = cone_terrain(W, H, period, offset)
+    note("cone basins %d offset %d" % (period, offset))
+    for preset in (1, 0):
+        op = oracle_params(oracle, preset)
+        hh = h.copy()
+        with fa.FastScape(W, H, gpu_params(preset), h=hh) as m:
+            for s in range(2):
+                st = oracle.step(hh, W, H, op, oracle.POW_LIBM)
+                m.setH(hh)
+                m.step(1)
+                if s == 0:
+                    assert 16 < st["levels"].size - 3 <= 32
+                    expect = {"halo16": 0, "halo32": 1 if halo32 else 0}
+                    assert m.tile_stats() == expect
+                    assert m.path_stats()["levels"] == (0 if halo32 else 1)
+                compare_step(m, st, "cone %d preset %d step %d" % (period, preset, s))
+                exact = oracle.step(hh, W, H, op, oracle.POW_EXACT)["h"]
+                assert_h(m.getH().ravel(), exact, st["h"], op.neq, "cone h step %d" % s)
+                hh = st["h"]
+
+
 def test_flat_and_nan_terrain(oracle):
     W, H = 50, 40
     op = oracle_params(oracle, 1)
@@ -280,10 +320,12 @@ def test_large_grid_invariants(oracle):
 
 
 @pytest.mark.parametrize("W,H,preset,steps", [(1000, 800, 1, 30), (130, 70, 1, 40),
-                                              (300, 300, 0, 60)])
+                                              (300, 300, 0, 60), (1000, 1000, 1, 500)])
 def test_tile_path_equals_level_path(oracle, W, H, preset, steps):
-    """The tile path and the level path are bit-identical step after step;
-    the reference preset deepens past the tile halo and falls back."""
+    """The tile path and the level path are bit-identical step after step.
+    The reference preset deepens past 16 levels within ten steps (32-cell
+    halo) and past 32 soon after (level path); the benchmark preset at 1000^2
+    passes 16 levels after about 350 steps and stays on the tile path."""
     note("tile vs levels %dx%d preset %d" % (W, H, preset))
     h0 = oracle.init_terrain(W, H, 0)
     with fa.FastScape(W, H, gpu_params(preset), h=h0) as a, \
@@ -300,10 +342,14 @@ def test_tile_path_equals_level_path(oracle, W, H, preset, steps):
         pa, pb = a.path_stats(), b.path_stats()
         assert pb == {"tile": 0, "levels": steps}
         assert pa["tile"] + pa["levels"] == steps
+        ta = a.tile_stats()
+        assert ta["halo16"] + ta["halo32"] == pa["tile"]
         if preset == 1:
             assert pa["tile"] == steps            # shallow drainage: always tiles
         else:
             assert pa["levels"] > 0                # deep drainage: falls back
+        if steps >= 60:
+            assert ta["halo16"] > 0 and ta["halo32"] > 0, ta
 
 
 def test_benchmark_grid_takes_tile_path(oracle):
