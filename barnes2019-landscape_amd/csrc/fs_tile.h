@@ -321,31 +321,35 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
   // passes, itself included; the thread's chains advance together, one cell
   // per round.  A walker stops at an outlet (its level is the round it got
   // there), where its chain leaves the region, or after kHalo hops (kHalo + 1
-  // rounds).  Walkers hold byte addresses into w_s.
+  // rounds).  Walkers hold byte addresses into w_s; a stopped walker holds
+  // the complement of the address of the cell that stopped it, so whether it
+  // reached an outlet is read back from that cell's code afterwards.
   constexpr int kPer = kRegCells / BLOCK;
   static_assert(kPer <= 16, "walkers per thread");
   int xa[kPer], hops[kPer];
-  uint32_t live = 0, rooted = 0;
+  uint32_t live = 0;
 #pragma unroll
   for (int j = 0; j < kPer; j++) {
     const int i = j * BLOCK + threadIdx.x;
-    xa[j] = 4 * i;
     hops[j] = 0;
-    if (acc_code(w_s[i]) != kStopOutside) live |= 1u << j;
+    if (acc_code(w_s[i]) != kStopOutside) {
+      xa[j] = 4 * i;
+      live |= 1u << j;
+    } else {
+      xa[j] = ~(4 * i);
+    }
   }
   for (int round = 0; round <= kHalo && live; round++) {
 #pragma unroll
     for (int j = 0; j < kPer; j++) {
-      if (!(live & (1u << j))) continue;
+      if (xa[j] < 0) continue;
       uint32_t *w = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(w_s) + xa[j]);
       const int code = acc_code(atomicAdd(w, 1u));
       if (code > kStopMax) {
         xa[j] += 4 * code;
       } else {
-        if (code == kStopRoot) {
-          rooted |= 1u << j;
-          hops[j] = round;
-        }
+        xa[j] = ~xa[j];
+        hops[j] = round;
         live &= ~(1u << j);
       }
     }
@@ -362,7 +366,8 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
     if (tx < 0 || tx >= kTile || ty < 0 || ty >= kTile) continue;
     if (acc_code(w_s[i]) == kStopOutside) continue;
     const size_t gi = static_cast<size_t>(y0 + ty) * W + (x0 + tx);
-    if (rooted & (1u << j)) {
+    const bool rooted = xa[j] < 0 && acc_code(w_s[~xa[j] >> 2]) == kStopRoot;
+    if (rooted) {
       lvl[gi] = static_cast<uint8_t>(hops[j]);
       if (level_hist) atomicAdd(&sh.hist[hops[j]], 1);
       deepest = max(deepest, hops[j]);
