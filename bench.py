@@ -80,6 +80,10 @@ def parse():
                     help="ensemble: steps per autotuner candidate (0 = no tuning)")
     ap.add_argument("--tune-interval", type=int, default=100)
     ap.add_argument("--size", type=int, default=10000)
+    ap.add_argument("--advance", type=int, default=0,
+                    help="grid workload: untimed steps before the warmup, so the timed window "
+                         "starts from a deeper drainage network (at 10000^2, 300 gives 17-32 "
+                         "levels: the 32-cell-halo tile path)")
     ap.add_argument("--profile-steps", type=int, default=10,
                     help="steps timed per kernel with HIP events for the roofline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -332,7 +336,7 @@ def main():
     W = H = args.size
     N = W * H
     m = fa.FastScape(W, H, fa.preset(fa.PRESET_BENCHMARK), engine_seed=0, device=0)
-    m.step(args.warmup)
+    m.step(args.advance + args.warmup)
     m.synchronize()
     t0 = time.perf_counter()
     m.step(args.steps)
@@ -348,6 +352,7 @@ def main():
     m.set_kernel_timing(False)
     levels = m.levels()
     paths = m.path_stats()
+    paths.update(m.tile_stats())
     n0 = int(levels[1]) if levels.size > 1 else N
     units = {"k_receivers": (W - 4) * (H - 4), "k_topology": N, "k_bfs": N - n0,
              "k_flowacc": N, "k_erode": N - n0, "k_tile_acc": N, "k_tile_erode": N}
@@ -356,7 +361,7 @@ def main():
     per_step_bytes = KERNEL_BYTES.get(dom, 0) * units.get(dom, N)
     achieved = per_step_bytes * args.profile_steps / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     traffic = None
-    if args.size == 10000 and dom in MEASURED_TRAFFIC_PER_CELL:
+    if args.size == 10000 and args.advance == 0 and dom in MEASURED_TRAFFIC_PER_CELL:
         traffic = MEASURED_TRAFFIC_PER_CELL[dom] * N
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -374,7 +379,8 @@ def main():
             "data": "synthetic (mt19937 seed 0 uniform[0,1) terrain, rings 0/1 at 0)",
             "config": {"workload": "FastScape %dx%d FP64 time-steps, K=1e-5 n=1 m=0.4 U=2e-3 "
                                    "dt=1000 dx=100" % (W, H),
-                       "grid": [W, H], "nlevel_last_step": int(levels.size),
+                       "grid": [W, H], "first_timed_step": args.advance + args.warmup,
+                       "nlevel_last_step": int(levels.size),
                        "step_paths": paths},
             "roofline": roofline, "cpu_baseline": cpu,
             "stage_kernel_ms_per_step": {k: round(v[0] / args.profile_steps, 4)
