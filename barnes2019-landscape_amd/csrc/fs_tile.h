@@ -227,10 +227,13 @@ __device__ __forceinline__ void load_region_words_x4(const uint8_t *__restrict__
 // Mark (bitmask over region cells) the halo cells on the receiver chains of
 // the tile's cells: walks start at tile cells whose receiver leaves the tile
 // and stop at an outlet, the region edge, a tile cell, an already marked cell
-// or after kHalo hops.  mark must be zero and w_s loaded before the call
-// (counts may still be changing: only the top byte and the tile bit are read).
+// or after kHalo hops.  Each newly marked cell is also appended to list (the
+// first kHaloCap of them; *nlist counts them all).  mark and *nlist must be
+// zero and w_s loaded before the call (counts may still be changing: only the
+// offset and the tile bit are read).
 template <int BLOCK>
-__device__ __forceinline__ void mark_halo_chains(const uint32_t *w_s, uint32_t *mark) {
+__device__ __forceinline__ void mark_halo_chains(const uint32_t *w_s, uint32_t *mark,
+                                                 uint16_t *list, uint32_t *nlist) {
   for (int it = 0; it < kTile * kTile / BLOCK; it++) {
     const int j = it * BLOCK + threadIdx.x;
     int c = (kHalo + j / kTile) * kReg + kHalo + j % kTile;
@@ -241,6 +244,8 @@ __device__ __forceinline__ void mark_halo_chains(const uint32_t *w_s, uint32_t *
       if (wd & kAccTile) break;
       const uint32_t bit = 1u << (c & 31);
       if (atomicOr(&mark[c >> 5], bit) & bit) break;   // marked by another walk
+      const uint32_t k = atomicAdd(nlist, 1u);
+      if (k < static_cast<uint32_t>(kHaloCap)) list[k] = static_cast<uint16_t>(c);
       code = acc_code(wd);
     }
   }
@@ -268,6 +273,8 @@ struct AccShared {
   uint32_t mark[kRegCells / 32];
   uint16_t pre[kRegCells / 32];   // marks before each mark word
   uint32_t scan[1];               // number of marks
+  uint32_t nlist;                 // marked cells appended to hlist
+  uint16_t hlist[kHaloCap];       // marked cells in marking order
   int over, maxlev;
   int hist[kHalo + 1];
 };
@@ -305,7 +312,10 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
   uint32_t *w_s = sh.w_s;
   const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
   const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
-  if (threadIdx.x == 0) sh.over = sh.maxlev = 0;
+  if (threadIdx.x == 0) {
+    sh.over = sh.maxlev = 0;
+    sh.nlist = 0;
+  }
   if (threadIdx.x < kHalo + 1) sh.hist[threadIdx.x] = 0;
   if (threadIdx.x < kRegCells / 32) sh.mark[threadIdx.x] = 0;
   bool done = false;
@@ -359,12 +369,23 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
   int over = 0, deepest = 0;
   constexpr int kJ0 = (kHalo * kReg) / BLOCK;
   constexpr int kJ1 = ((kHalo + kTile) * kReg - 1) / BLOCK;
+  // region column and row of slot j's cell, stepped from slot kJ0's
+  constexpr int kStepY = BLOCK / kReg, kStepX = BLOCK % kReg;
+  int lx = (kJ0 * BLOCK + static_cast<int>(threadIdx.x)) % kReg;
+  int ly = (kJ0 * BLOCK + static_cast<int>(threadIdx.x)) / kReg;
 #pragma unroll
   for (int j = kJ0; j <= kJ1; j++) {
-    const int i = j * BLOCK + threadIdx.x;
-    const int tx = i % kReg - kHalo, ty = i / kReg - kHalo;
-    if (tx < 0 || tx >= kTile || ty < 0 || ty >= kTile) continue;
-    if (acc_code(w_s[i]) == kStopOutside) continue;
+    const int tx = lx - kHalo, ty = ly - kHalo;
+    lx += kStepX;
+    ly += kStepY;
+    if (lx >= kReg) {
+      lx -= kReg;
+      ly++;
+    }
+    if (static_cast<unsigned>(tx) >= static_cast<unsigned>(kTile) ||
+        static_cast<unsigned>(ty) >= static_cast<unsigned>(kTile))
+      continue;
+    if (x0 + tx >= W || y0 + ty >= H) continue;     // beyond the grid
     const size_t gi = static_cast<size_t>(y0 + ty) * W + (x0 + tx);
     const bool rooted = xa[j] < 0 && acc_code(w_s[~xa[j] >> 2]) == kStopRoot;
     if (rooted) {
@@ -381,7 +402,7 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
     for (int d = 1; d < kWave; d <<= 1) deepest = max(deepest, __shfl_xor(deepest, d));
     if ((threadIdx.x & (kWave - 1)) == 0 && deepest > 0) atomicMax(&sh.maxlev, deepest);
   }
-  mark_halo_chains<BLOCK>(w_s, sh.mark);
+  mark_halo_chains<BLOCK>(w_s, sh.mark, sh.hlist, &sh.nlist);
   __syncthreads();
   // halo slot bases: one wave scans the mark words, kMarkRun per lane
   constexpr int kMarkWords = kRegCells / 32;
@@ -415,16 +436,21 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
     atomicAdd(&level_hist[threadIdx.x], sh.hist[threadIdx.x]);
   if (!tile_over) {
     // the plan: every needed cell's receiver slot; halo cells' region index
-    // and level
-#pragma unroll
-    for (int j = 0; j < kPer; j++) {
-      const int i = j * BLOCK + threadIdx.x;
-      const uint32_t mw = sh.mark[i >> 5];
-      if (!(mw & (1u << (i & 31)))) continue;
-      const int s = kTileCells + sh.pre[i >> 5] + __popc(mw & ((1u << (i & 31)) - 1u));
-      plan->rslot[s] = plan_receiver(w_s, sh.mark, sh.pre, i);
-      plan->hreg[s - kTileCells] = static_cast<uint16_t>(i);
-      plan->hlvl[s - kTileCells] = static_cast<uint8_t>(hops[j]);
+    // and level (the hops from the cell to its outlet, all inside the region
+    // since the tile is not over)
+    static_assert(kHaloCap <= BLOCK, "one pass over the marked cells");
+    if (threadIdx.x < nhalo) {
+      const int c = sh.hlist[threadIdx.x];
+      const int s = kTileCells + sh.pre[c >> 5] + __popc(sh.mark[c >> 5] & ((1u << (c & 31)) - 1u));
+      plan->rslot[s] = plan_receiver(w_s, sh.mark, sh.pre, c);
+      plan->hreg[s - kTileCells] = static_cast<uint16_t>(c);
+      int lev = 0, cc = c, code = acc_code(w_s[c]);
+      while (code > kStopMax && lev <= kHalo) {
+        cc += code;
+        code = acc_code(w_s[cc]);
+        lev++;
+      }
+      plan->hlvl[s - kTileCells] = static_cast<uint8_t>(lev);
     }
     for (int it = 0; it < kTileCells / BLOCK; it++) {
       const int tj = it * BLOCK + threadIdx.x;
