@@ -275,6 +275,7 @@ struct AccShared {
   uint32_t scan[1];               // number of marks
   uint32_t nlist;                 // marked cells appended to hlist
   uint16_t hlist[kHaloCap];       // marked cells in marking order
+  uint8_t lv_t[kTileCells];       // level of each tile cell, stored with the counts
   int over, maxlev;
   int hist[kHalo + 1];
 };
@@ -386,10 +387,9 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
         static_cast<unsigned>(ty) >= static_cast<unsigned>(kTile))
       continue;
     if (x0 + tx >= W || y0 + ty >= H) continue;     // beyond the grid
-    const size_t gi = static_cast<size_t>(y0 + ty) * W + (x0 + tx);
     const bool rooted = xa[j] < 0 && acc_code(w_s[~xa[j] >> 2]) == kStopRoot;
+    sh.lv_t[ty * kTile + tx] = static_cast<uint8_t>(hops[j]);   // a level when rooted
     if (rooted) {
-      lvl[gi] = static_cast<uint8_t>(hops[j]);
       if (level_hist) atomicAdd(&sh.hist[hops[j]], 1);
       deepest = max(deepest, hops[j]);
     } else {
@@ -481,7 +481,9 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
     const int tx = i % kTile, ty = i / kTile;
     const int gx = x0 + tx, gy = y0 + ty;
     if (gx >= W || gy >= H) continue;
-    cnt[static_cast<size_t>(gy) * W + gx] = w_s[(ty + kHalo) * kReg + tx + kHalo] & kAccCount;
+    const size_t gi = static_cast<size_t>(gy) * W + gx;
+    cnt[gi] = w_s[(ty + kHalo) * kReg + tx + kHalo] & kAccCount;
+    lvl[gi] = sh.lv_t[i];
   }
   return tile_over;
 }
