@@ -72,6 +72,8 @@ struct fs_model {
   ErodeConst ek{};
   int nmode = 1;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;     // header read-back of the tile path
+  hipEvent_t acc_done = nullptr;  // tile accumulation finished on `stream`
 
   // device state; h and h_alt swap after each tile-path step
   double *h = nullptr, *h_alt = nullptr;
@@ -210,6 +212,8 @@ void free_model(fs_model *m) {
     (void)hipEventDestroy(t.b);
   }
   for (auto e : m->kfree) (void)hipEventDestroy(e);
+  if (m->acc_done) (void)hipEventDestroy(m->acc_done);
+  if (m->side) (void)hipStreamDestroy(m->side);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
 }
@@ -241,6 +245,8 @@ int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_
   m->ek = make_erode_const(*p, &m->nmode);
   FS_HIP_TRY(hipSetDevice(device));
   FS_HIP_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+  FS_HIP_TRY(hipStreamCreateWithFlags(&m->side, hipStreamNonBlocking));
+  FS_HIP_TRY(hipEventCreateWithFlags(&m->acc_done, hipEventDisableTiming));
   for (auto &set : m->ev)
     for (auto &e : set) FS_HIP_TRY(hipEventCreate(&e));
 
@@ -376,7 +382,7 @@ int launch_receivers(fs_model *m) {
   const unsigned gy = static_cast<unsigned>(std::min((m->H - 4 + kRecvRows - 1) / kRecvRows, 65535));
   FS_TRY(kt_begin(m, FS_K_RECEIVERS));
   hipLaunchKernelGGL(k_receivers, dim3(gx, gy), dim3(256), 0, m->stream, m->h, m->rcv, m->W, m->H,
-                     m->d_card, m->d_diag);
+                     m->d_card, m->d_diag, m->hdr);
   FS_TRY(check_launch());
   return kt_end(m);
 }
@@ -543,16 +549,22 @@ int launch_tile_erode(fs_model *m, int tiles, int tiles_x, int halo) {
   return check_launch();
 }
 
-// Accumulation with the given halo (16 or 32) and the header read after it.
-// Returns FS_OK with *fits = false when some level exceeds the halo (or a
-// tile's chains need more halo cells than its plan holds); hdr_h[0] is then
-// the deepest level among the cells that reach an outlet within the region.
-int tile_accumulate(fs_model *m, int halo, int tiles, int tiles_x, hipEvent_t ev_done,
-                    bool *fits) {
+// Accumulation with the given halo (16 or 32), then the erosion with the
+// same halo launched behind it at once: the erosion grid does nothing when a
+// tile was too deep, so the host reads the accumulation's header on a side
+// stream while the erosion runs.  *fits = false when some level exceeds the
+// halo (or a tile's chains need more halo cells than its plan holds); the
+// erosion was then empty and hdr_h[0] is the deepest level among the cells
+// that reach an outlet within the region.  clear: reset the header first
+// (k_receivers already did for the first accumulation of a step).
+int tile_attempt(fs_model *m, int halo, int tiles, int tiles_x, bool clear, hipEvent_t *ev,
+                 bool *fits) {
   hipStream_t s = m->stream;
   *fits = false;
-  hipLaunchKernelGGL(k_fill_counts, dim3(1), dim3(64), 0, s, m->hdr, 0);
-  FS_TRY(check_launch());
+  if (clear) {
+    hipLaunchKernelGGL(k_fill_counts, dim3(1), dim3(64), 0, s, m->hdr, 0);
+    FS_TRY(check_launch());
+  }
   FS_TRY(kt_begin(m, FS_K_TILE_ACC));
   if (halo == kHalo) {
     hipLaunchKernelGGL(k_tile_acc, dim3(tiles), dim3(kAccBlock), 0, s, m->rcv, m->cnt, m->lvl,
@@ -564,39 +576,11 @@ int tile_accumulate(fs_model *m, int halo, int tiles, int tiles_x, hipEvent_t ev
   }
   FS_TRY(check_launch());
   FS_TRY(kt_end(m));
-  FS_HIP_TRY(hipEventRecord(ev_done, s));
-  FS_TRY(fetch_header(m, 0));
-  FS_TRY(device_error_status(m->hdr_h[2]));
-  *fits = m->hdr_h[3] == 0;
-  return FS_OK;
-}
-
-// The tile path (every level <= 32): accumulation with a 16-cell halo, or a
-// 32-cell one when the drainage is deeper, then erosion with the same halo,
-// one header read in between.  Returns FS_OK with *done = false when the
-// levels are too deep for either; the caller then runs the level path on the
-// same receivers.  The halo that fitted is tried first at the next step.
-int tile_path(fs_model *m, hipEvent_t *ev, bool *done) {
-  *done = false;
-  hipStream_t s = m->stream;
-  const int tiles_x = (m->W + kTile - 1) / kTile, tiles_y = (m->H + kTile - 1) / kTile;
-  const int tiles = tiles_x * tiles_y;
-  int halo = m->tile_halo == tile32::kHalo ? tile32::kHalo : kHalo;
-  bool fits = false;
-  FS_TRY(tile_accumulate(m, halo, tiles, tiles_x, ev[2], &fits));
-  FS_TRY(stage_collect(m, m->ev_cur ^ 1));
-  FS_TRY(kt_collect(m));
-  if (!fits && halo == kHalo) {
-    halo = tile32::kHalo;
-    FS_TRY(tile_accumulate(m, halo, tiles, tiles_x, ev[2], &fits));
-  }
-  if (!fits) {
-    m->tile_halo = kHalo;                   // the level path sets it from the depth it finds
-    return FS_OK;
-  }
-  // a 32-halo step whose levels all fit 16 tries the 16-cell halo next
-  m->tile_halo = (halo == tile32::kHalo && m->hdr_h[0] <= kHalo) ? kHalo : halo;
-  m->order_current = false;                 // levels, stack, donor masks on request
+  FS_HIP_TRY(hipEventRecord(ev[2], s));
+  FS_HIP_TRY(hipEventRecord(m->acc_done, s));
+  FS_HIP_TRY(hipStreamWaitEvent(m->side, m->acc_done, 0));
+  FS_HIP_TRY(hipMemcpyAsync(m->hdr_h, m->hdr, kHdrInts * sizeof(int32_t), hipMemcpyDeviceToHost,
+                            m->side));
   FS_HIP_TRY(hipEventRecord(ev[3], s));
   FS_HIP_TRY(hipEventRecord(ev[4], s));
   FS_HIP_TRY(hipEventRecord(ev[5], s));
@@ -605,6 +589,37 @@ int tile_path(fs_model *m, hipEvent_t *ev, bool *done) {
   else if (m->nmode == 2) FS_TRY(launch_tile_erode<2>(m, tiles, tiles_x, halo));
   else FS_TRY(launch_tile_erode<0>(m, tiles, tiles_x, halo));
   FS_TRY(kt_end(m));
+  FS_HIP_TRY(hipStreamSynchronize(m->side));
+  FS_TRY(device_error_status(m->hdr_h[2]));
+  *fits = m->hdr_h[3] == 0;
+  return FS_OK;
+}
+
+// The tile path (every level <= 32): accumulation with a 16-cell halo, or a
+// 32-cell one when the drainage is deeper, and erosion with the same halo.
+// Returns FS_OK with *done = false when the levels are too deep for either
+// (both erosions were empty); the caller then runs the level path on the
+// same receivers.  The halo that fitted is tried first at the next step.
+int tile_path(fs_model *m, hipEvent_t *ev, bool *done) {
+  *done = false;
+  const int tiles_x = (m->W + kTile - 1) / kTile, tiles_y = (m->H + kTile - 1) / kTile;
+  const int tiles = tiles_x * tiles_y;
+  int halo = m->tile_halo == tile32::kHalo ? tile32::kHalo : kHalo;
+  bool fits = false;
+  FS_TRY(tile_attempt(m, halo, tiles, tiles_x, false, ev, &fits));
+  FS_TRY(stage_collect(m, m->ev_cur ^ 1));
+  FS_TRY(kt_collect(m));
+  if (!fits && halo == kHalo) {
+    halo = tile32::kHalo;
+    FS_TRY(tile_attempt(m, halo, tiles, tiles_x, true, ev, &fits));
+  }
+  if (!fits) {
+    m->tile_halo = kHalo;                   // the level path sets it from the depth it finds
+    return FS_OK;
+  }
+  // a 32-halo step whose levels all fit 16 tries the 16-cell halo next
+  m->tile_halo = (halo == tile32::kHalo && m->hdr_h[0] <= kHalo) ? kHalo : halo;
+  m->order_current = false;                 // levels, stack, donor masks on request
   std::swap(m->h, m->h_alt);
   m->tile_steps++;
   if (halo != kHalo) m->tile32_steps++;

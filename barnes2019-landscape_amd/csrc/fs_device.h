@@ -205,9 +205,17 @@ __device__ __forceinline__ void receivers_strip(const double *__restrict__ h,
   }
 }
 
+// Also clears the header words the tile path's accumulation reports into
+// (hdr[0], hdr[1], hdr[3]; as k_fill_counts with no level counts).
 __global__ __launch_bounds__(256) void k_receivers(const double *__restrict__ h,
                                                    uint8_t *__restrict__ rcv, int W, int H,
-                                                   double d_card, double d_diag) {
+                                                   double d_card, double d_diag,
+                                                   int32_t *__restrict__ hdr) {
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    hdr[0] = 0;
+    hdr[1] = 0;
+    hdr[3] = 0;
+  }
   const int x = blockIdx.x * 256 + threadIdx.x + 2;
   if (x >= W - 2) return;
   const int nstrips = (H - 4 + kRecvRows - 1) / kRecvRows;
