@@ -148,8 +148,15 @@ int kt_end(fs_model *m) {
 }
 
 // All recorded kernel events are complete (called after a stream sync).
+// Fold the timed launches that have finished (in launch order; the tile
+// path's erosion may still be running when this is called).
 int kt_collect(fs_model *m) {
-  for (auto &t : m->kpending) {
+  size_t done = 0;
+  for (; done < m->kpending.size(); done++) {
+    const KTimed &t = m->kpending[done];
+    const hipError_t q = hipEventQuery(t.b);
+    if (q == hipErrorNotReady) break;
+    FS_HIP_TRY(q);
     float ms = 0.f;
     FS_HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
     m->kms[t.kid] += ms;
@@ -157,7 +164,7 @@ int kt_collect(fs_model *m) {
     m->kfree.push_back(t.a);
     m->kfree.push_back(t.b);
   }
-  m->kpending.clear();
+  m->kpending.erase(m->kpending.begin(), m->kpending.begin() + static_cast<std::ptrdiff_t>(done));
   return FS_OK;
 }
 

@@ -365,6 +365,33 @@ def test_benchmark_grid_takes_tile_path(oracle):
         np.testing.assert_array_equal(m.getH().ravel(), st["h"])
 
 
+def test_kernel_and_stage_timing_on_both_paths(oracle):
+    """Per-kernel HIP-event timing and the stage timers while the tile path
+    runs (its erosion is still in flight when the host collects) and on the
+    level path; timing does not change the results."""
+    note("kernel timing 300x260")
+    W, H = 300, 260
+    h0 = oracle.init_terrain(W, H, 0)
+    with fa.FastScape(W, H, gpu_params(1), h=h0) as a, \
+            fa.FastScape(W, H, gpu_params(1), h=h0) as b:
+        a.set_kernel_timing(True)
+        a.step(6)
+        b.step(6)
+        st = a.kernel_stats()
+        for k in ("k_receivers", "k_tile_acc", "k_tile_erode"):
+            assert st[k][1] == 6 and st[k][0] > 0, (k, st[k])
+        assert a.path_stats() == {"tile": 6, "levels": 0}
+        np.testing.assert_array_equal(a.getH(), b.getH())
+        a.set_path(fa.PATH_LEVELS)
+        a.set_kernel_timing(True)
+        a.step(2)
+        st = a.kernel_stats()
+        assert st["k_receivers"][1] == 2 and st["k_erode"][1] + st["k_uplift_roots"][1] > 0
+        assert st["k_tile_acc"][1] == 0
+        t = a.timers()
+        assert all(v >= 0 for v in t.values()) and max(t.values()) > 0, t
+
+
 @pytest.mark.parametrize("W,H", [(300, 260), (130, 70)])
 def test_receivers_of_consecutive_tile_steps(oracle, W, H):
     """Consecutive tile-path steps without set_h (the tile path alternates two
