@@ -84,10 +84,12 @@ __device__ __forceinline__ bool in_tile(int lx, int ly) {
 }
 
 // Accumulation word of a region cell in LDS: bits 0-14 its count, bit 15
-// set for a tile cell, bits 16-31 the signed region-index offset of its
-// receiver or a stop code.  One atomic add both counts a walker's visit and
-// tells it where to go next.  The stop codes lie below every offset (-128..-126
-// for a 16-cell halo, so that they also fit a byte).
+// set for a tile cell, bits 16-31 four times the signed region-index offset
+// of its receiver (the byte offset in the word array) or of a stop code.  One
+// atomic add both counts a walker's visit and tells it where to go next: the
+// returned word shifted right by 16 is the step of the walker's byte address.
+// The stop codes lie below every offset (-128..-126 for a 16-cell halo, so
+// that they also fit a byte).
 constexpr uint32_t kAccCount = 0x7FFFu;
 constexpr uint32_t kAccTile = 1u << 15;
 constexpr int kStopRoot = -(kReg + 32);        // an outlet (NO_FLOW)
@@ -99,9 +101,13 @@ static_assert(kRegCells <= static_cast<int>(kAccCount) && kReg + 1 < -kStopMax,
 // offsets and stop codes fit one signed byte: the dword loader's lookups apply
 constexpr bool kByteCodes = kStopRoot >= -128;
 
-__device__ __forceinline__ int acc_code(uint32_t w) { return static_cast<int32_t>(w) >> 16; }
+static_assert(4 * kStopRoot >= -32768, "byte offsets fit the word's top half");
+// words above this (as signed integers) carry an offset, not a stop code
+constexpr int32_t kStopWordMax = static_cast<int32_t>(
+    static_cast<uint32_t>(4 * kStopMax) << 16 | 0xFFFFu);
+__device__ __forceinline__ int acc_code(uint32_t w) { return static_cast<int32_t>(w) >> 18; }
 __device__ __forceinline__ uint32_t acc_word(int code, bool tile) {
-  return static_cast<uint32_t>(code & 0xFFFF) << 16 | (tile ? kAccTile : 0u);
+  return static_cast<uint32_t>((4 * code) & 0xFFFF) << 16 | (tile ? kAccTile : 0u);
 }
 
 // Load the receivers of the region into accumulation words with zero counts
@@ -214,12 +220,12 @@ __device__ __forceinline__ void load_region_words_x4(const uint8_t *__restrict__
     const bool tile = static_cast<unsigned>(row - kHalo) < static_cast<unsigned>(kTile) &&
                       static_cast<unsigned>(col - kHalo / 4) < static_cast<unsigned>(kTile / 4);
     const uint32_t t = tile ? kAccTile : 0u;
-    // byte b of code, sign-extended, into bits 16-31 of word b
+    // byte b of code, sign-extended and times four, into bits 16-31 of word b
     uint4 q;
-    q.x = (static_cast<uint32_t>(static_cast<int32_t>(code << 24) >> 8) & 0xFFFF0000u) | t;
-    q.y = (static_cast<uint32_t>(static_cast<int32_t>(code << 16) >> 8) & 0xFFFF0000u) | t;
-    q.z = (static_cast<uint32_t>(static_cast<int32_t>(code << 8) >> 8) & 0xFFFF0000u) | t;
-    q.w = (static_cast<uint32_t>(static_cast<int32_t>(code) >> 8) & 0xFFFF0000u) | t;
+    q.x = (static_cast<uint32_t>(static_cast<int32_t>(code << 24) >> 6) & 0xFFFC0000u) | t;
+    q.y = (static_cast<uint32_t>(static_cast<int32_t>(code << 16) >> 6) & 0xFFFC0000u) | t;
+    q.z = (static_cast<uint32_t>(static_cast<int32_t>(code << 8) >> 6) & 0xFFFC0000u) | t;
+    q.w = (static_cast<uint32_t>(static_cast<int32_t>(code) >> 6) & 0xFFFC0000u) | t;
     *reinterpret_cast<uint4 *>(&w_s[4 * d]) = q;
   }
 }
@@ -355,9 +361,9 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
     for (int j = 0; j < kPer; j++) {
       if (xa[j] < 0) continue;
       uint32_t *w = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(w_s) + xa[j]);
-      const int code = acc_code(atomicAdd(w, 1u));
-      if (code > kStopMax) {
-        xa[j] += 4 * code;
+      const int32_t wd = static_cast<int32_t>(atomicAdd(w, 1u));
+      if (wd > kStopWordMax) {
+        xa[j] += wd >> 16;
       } else {
         xa[j] = ~xa[j];
         hops[j] = round;
