@@ -39,3 +39,16 @@ def oracle():
 
 def params_for(o, preset):
     return o.preset(int(preset))
+
+
+def cone_terrain(W, H, period, offset):
+    """Conical basins `period` cells wide: each cell drains to its basin's
+    centre, up to period/2 receiver hops away (48-cell basins: 25 levels)."""
+    x = np.arange(W, dtype=np.float64)[None, :]
+    y = np.arange(H, dtype=np.float64)[:, None]
+    cx = (x + offset) % period - period / 2 + 0.5
+    cy = (y + offset) % period - period / 2 + 0.5
+    h = 1.0 + np.sqrt(cx * cx + cy * cy) + 1e-7 * ((x * 7 + y * 13) % 11)
+    h[:2, :] = h[-2:, :] = 0
+    h[:, :2] = h[:, -2:] = 0
+    return np.ascontiguousarray(h).ravel()

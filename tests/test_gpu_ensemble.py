@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import fastscape_amd as fa
+from conftest import cone_terrain
 
 pytestmark = pytest.mark.gpu
 
@@ -174,6 +175,26 @@ def test_set_h_resume_and_errors():
             ens.setH(0, np.zeros(7))
         with pytest.raises(fa.FastScapeError):
             ens.set_schedule(fa.schedule(recv_rows=20))
+
+
+def test_ensemble_level_path_matches_deep_tile_halo():
+    """A realization 17-32 levels deep (48-cell conical basins, 25 levels):
+    the ensemble runs it on its level path, the single-grid model on the tile
+    path with the 32-cell halo; both give the same bits at every stage."""
+    note("ensemble with a 25-level realization")
+    cone = cone_terrain(200, 160, 48, 0)
+    specs = [(200, 160, bench(), 0), (64, 64, bench(), 1)]
+    with make(specs) as ens, fa.FastScape(200, 160, bench(), h=cone) as m, \
+            fa.FastScape(64, 64, bench(), engine_seed=1) as m1:
+        ens.setH(0, cone)
+        for step in range(4):
+            ens.step()
+            m.step(1)
+            m1.step(1)
+            compare_all(ens, [m, m1], specs, "cone step %d" % step)
+            assert 16 < ens.levels(0).size - 3 <= 32
+        assert ens.path_stats(0) == (0, 4)
+        assert m.tile_stats()["halo32"] >= 1
 
 
 def test_scaled_config4_sample():

@@ -22,7 +22,8 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import FINAL_FIXTURES, ROOT, STAGE_FIXTURES, golden_dem_shas, load_golden
+from conftest import (FINAL_FIXTURES, ROOT, STAGE_FIXTURES, cone_terrain, golden_dem_shas,
+                      load_golden)
 
 import fastscape_amd as fa
 
@@ -189,19 +190,6 @@ def test_deep_drainage_vs_oracle(oracle, kind):
                 exact = oracle.step(hh, W, H, op, oracle.POW_EXACT)["h"]
                 assert_h(m.getH().ravel(), exact, st["h"], op.neq, "%s h" % kind)
                 hh = st["h"]
-
-
-def cone_terrain(W, H, period, offset):
-    """Conical basins `period` cells wide: each cell drains to its basin's
-    centre, up to period/2 receiver hops away."""
-    x = np.arange(W, dtype=np.float64)[None, :]
-    y = np.arange(H, dtype=np.float64)[:, None]
-    cx = (x + offset) % period - period / 2 + 0.5
-    cy = (y + offset) % period - period / 2 + 0.5
-    h = 1.0 + np.sqrt(cx * cx + cy * cy) + 1e-7 * ((x * 7 + y * 13) % 11)
-    h[:2, :] = h[-2:, :] = 0
-    h[:, :2] = h[:, -2:] = 0
-    return np.ascontiguousarray(h).ravel()
 
 
 # 48-cell basins: 25 levels and at most 460 halo cells on any tile's chains,
