@@ -738,7 +738,7 @@ __global__ __launch_bounds__(kEroBlock) void k_tile_erode(
     int tiles_x, int ntiles, ErodeConst k, int32_t *hdr) {
   __shared__ EroShared sh;
   // launched right behind the accumulation, before the host has read its
-  // verdict: when a tile was too deep (hdr[3], the plans are incomplete)
+  // outcome: when a tile was too deep (hdr[3], the plans are incomplete)
   // the whole grid does nothing
   if (hdr[3] != 0) return;
   const int t = xcd_tile(blockIdx.x, ntiles);
