@@ -272,7 +272,7 @@ int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_
   // at least kFactCache counts, so the tile kernels' LDS cache stays in bounds
   const size_t kmax = std::max<size_t>(N, kFactCache - 1);
   FS_TRY(fs::dev_alloc(&m->ftab, 2 * (kmax + 1)));
-  FS_TRY(fs::dev_alloc(&m->hdr, kHdrInts + FS_MAX_LEVELS + 3));
+  FS_TRY(fs::dev_alloc(&m->hdr, kHdrAlloc));
   FS_TRY(fs::dev_alloc(&m->levels_d, FS_MAX_LEVELS + 4));
   FS_TRY(fs::dev_alloc(&m->status, ntiles));
   FS_TRY(fs::dev_alloc(&m->ticket, 1));
@@ -287,7 +287,7 @@ int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_
     return FS_ERR_OUT_OF_MEMORY;
   }
   FS_HIP_TRY(hipMemsetAsync(m->rcv, 0xFF, N + 16, m->stream));   // RB:361-362
-  FS_HIP_TRY(hipMemsetAsync(m->hdr, 0, (kHdrInts + FS_MAX_LEVELS + 3) * sizeof(int32_t), m->stream));
+  FS_HIP_TRY(hipMemsetAsync(m->hdr, 0, kHdrAlloc * sizeof(int32_t), m->stream));
   FS_HIP_TRY(hipMemsetAsync(m->status, 0, ntiles * sizeof(uint64_t), m->stream));
   FS_HIP_TRY(hipMemsetAsync(m->ticket, 0, sizeof(unsigned long long), m->stream));
   FS_HIP_TRY(hipMemsetAsync(m->plan, 0, plan_tiles * sizeof(TilePlan), m->stream));
@@ -449,7 +449,7 @@ int build_order(fs_model *m, hipEvent_t ev_topology_done) {
     for (int pass = 0; pass < kJumpPasses; pass++) {
       FS_TRY(kt_begin(m, FS_K_JUMP));
       hipLaunchKernelGGL(k_jump, dim3(kJumpBlocks), dim3(256), 0, s, m->list, m->depth, m->jump,
-                         m->hdr);
+                         m->hdr, pass);
       FS_TRY(check_launch());
       FS_TRY(kt_end(m));
     }

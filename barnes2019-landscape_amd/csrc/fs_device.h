@@ -31,8 +31,13 @@ constexpr int kSmallLevel = kSmallBlock * kSmallItems;  // one-workgroup levels
 constexpr int kNewtonCap = 100;
 constexpr int kHdrInts = 4;            // maxdepth, n_unfinished, err, spare
 constexpr int kHdrCounts = 1024;       // level counts fetched with the header
+// After the level counts: one "cells still unfinished" flag per pointer-
+// jumping pass, so the passes after the last needed one return at once.
+constexpr int kHdrJump = kHdrInts + FS_MAX_LEVELS + 3;
+constexpr int kHdrAlloc = kHdrJump + kJumpPasses + 1;   // header ints allocated
 constexpr int kTopoBlocks = 2048;
 constexpr int kJumpBlocks = 1024;
+constexpr int kJumpHist = 2048;         // levels k_jump counts in LDS
 
 // device error bits (hdr[2])
 constexpr int kErrLookback = 1, kErrNewton = 2, kErrIndex = 4, kErrDepth = 8;
@@ -174,6 +179,7 @@ __global__ void k_fill_counts(int32_t *hdr, int n_counts) {
     hdr[3] = 0;   // tiles whose levels exceed the tile halo
   }
   if (i < n_counts) hdr[kHdrInts + i] = 0;
+  if (i <= kJumpPasses) hdr[kHdrJump + i] = 0;
 }
 
 // ComputeReceivers (RB:160-182).  Each thread owns one column of a strip of
@@ -287,12 +293,26 @@ __global__ __launch_bounds__(256) void k_topology(
 // whose recorded ancestor has a level takes level = hops + that level;
 // otherwise it jumps to its ancestor's ancestor.  (ancestor, hops) pairs are
 // read and written as one 64-bit word, so every pair a thread sees is
-// consistent and the update is safe in place.
+// consistent and the update is safe in place.  A pass that leaves a cell
+// unfinished raises the next pass's flag; a pass whose flag is down (the
+// previous pass finished every cell) returns at once.
 __global__ __launch_bounds__(256) void k_jump(const int32_t *__restrict__ list,
                                               uint16_t *depth, unsigned long long *jump,
-                                              int32_t *hdr) {
+                                              int32_t *hdr, int pass) {
+  // level counts below kJumpHist and the deepest level are folded per block
+  // in LDS: finished cells share a few hundred levels, so global atomics per
+  // cell would queue on the same words
+  __shared__ int hist[kJumpHist];
+  __shared__ int smax, smore;
+  if (pass > 0 &&
+      __hip_atomic_load(&hdr[kHdrJump + pass], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    return;
+  for (int i = threadIdx.x; i < kJumpHist; i += blockDim.x) hist[i] = 0;
+  if (threadIdx.x == 0) smax = smore = 0;
+  __syncthreads();
   const int n = hdr[1];
   int32_t *counts = hdr + kHdrInts;
+  int local_max = 0, more = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int c = list[i];
     if (__hip_atomic_load(&depth[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0xFFFF)
@@ -310,8 +330,9 @@ __global__ __launch_bounds__(256) void k_jump(const int32_t *__restrict__ list,
       }
       __hip_atomic_store(&depth[c], static_cast<uint16_t>(nd), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
-      atomicAdd(&counts[nd], 1);
-      atomicMax(&hdr[0], static_cast<int>(nd));
+      if (nd < static_cast<unsigned>(kJumpHist)) atomicAdd(&hist[nd], 1);
+      else atomicAdd(&counts[nd], 1);
+      local_max = max(local_max, static_cast<int>(nd));
     } else {
       const unsigned long long ja =
           __hip_atomic_load(&jump[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -323,7 +344,20 @@ __global__ __launch_bounds__(256) void k_jump(const int32_t *__restrict__ list,
       __hip_atomic_store(&jump[c],
                          (static_cast<unsigned long long>(nd) << 32) | (ja & 0xffffffffu),
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      more = 1;
     }
+  }
+  if (local_max > 0) atomicMax(&smax, local_max);
+  if (more) smore = 1;
+  __syncthreads();
+  const int top = min(smax, kJumpHist - 1);
+  for (int i = threadIdx.x; i <= top; i += blockDim.x)
+    if (hist[i]) atomicAdd(&counts[i], hist[i]);
+  if (threadIdx.x == 0) {
+    if (smax > 0) atomicMax(&hdr[0], smax);
+    if (smore)
+      __hip_atomic_store(&hdr[kHdrJump + pass + 1], 1, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
