@@ -80,6 +80,10 @@ def parse():
                     help="ensemble: steps per autotuner candidate (0 = no tuning)")
     ap.add_argument("--tune-interval", type=int, default=100)
     ap.add_argument("--size", type=int, default=10000)
+    ap.add_argument("--preset", choices=["benchmark", "reference"], default="benchmark",
+                    help="grid workload parameters: the benchmark preset (K=1e-5, n=1, m=0.4, "
+                         "dx=100) or the reference's own (K=2e-6, n=2, m=0.8, dx=1), whose "
+                         "drainage deepens past the tile halos within tens of steps")
     ap.add_argument("--advance", type=int, default=0,
                     help="grid workload: untimed steps before the warmup, so the timed window "
                          "starts from a deeper drainage network (at 10000^2, 300 gives 17-32 "
@@ -335,7 +339,8 @@ def main():
     args.steps = args.steps or 121
     W = H = args.size
     N = W * H
-    m = fa.FastScape(W, H, fa.preset(fa.PRESET_BENCHMARK), engine_seed=0, device=0)
+    which = fa.PRESET_BENCHMARK if args.preset == "benchmark" else fa.PRESET_REFERENCE
+    m = fa.FastScape(W, H, fa.preset(which), engine_seed=0, device=0)
     m.step(args.advance + args.warmup)
     m.synchronize()
     t0 = time.perf_counter()
@@ -361,7 +366,8 @@ def main():
     per_step_bytes = KERNEL_BYTES.get(dom, 0) * units.get(dom, N)
     achieved = per_step_bytes * args.profile_steps / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     traffic = None
-    if args.size == 10000 and args.advance == 0 and dom in MEASURED_TRAFFIC_PER_CELL:
+    if (args.size == 10000 and args.advance == 0 and args.preset == "benchmark"
+            and dom in MEASURED_TRAFFIC_PER_CELL):
         traffic = MEASURED_TRAFFIC_PER_CELL[dom] * N
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -370,15 +376,17 @@ def main():
     m.close()
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.preset == "benchmark":   # timed on that preset
         cpu = cpu_baseline(args.cpu_sample_size, args.cpu_sample_run)
 
     line = {"metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": secs * 1e3 / args.steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (mt19937 seed 0 uniform[0,1) terrain, rings 0/1 at 0)",
-            "config": {"workload": "FastScape %dx%d FP64 time-steps, K=1e-5 n=1 m=0.4 U=2e-3 "
-                                   "dt=1000 dx=100" % (W, H),
+            "config": {"workload": ("FastScape %dx%d FP64 time-steps, K=1e-5 n=1 m=0.4 U=2e-3 "
+                                    "dt=1000 dx=100" if args.preset == "benchmark" else
+                                    "FastScape %dx%d FP64 time-steps, reference preset K=2e-6 "
+                                    "n=2 m=0.8 U=2e-3 dt=1000 dx=1") % (W, H),
                        "grid": [W, H], "first_timed_step": args.advance + args.warmup,
                        "nlevel_last_step": int(levels.size),
                        "step_paths": paths},
