@@ -147,7 +147,6 @@ int kt_end(fs_model *m) {
   return FS_OK;
 }
 
-// All recorded kernel events are complete (called after a stream sync).
 // Fold the timed launches that have finished (in launch order; the tile
 // path's erosion may still be running when this is called).
 int kt_collect(fs_model *m) {
