@@ -62,15 +62,17 @@ int main(int argc, char **argv) {
     std::cout << "t Total calculation time    = " << std::setw(15) << us << " microseconds" << std::endl;
     // metadata lines (tag m: a column for d.make_tabular) -- no reference
     // counterpart: throughput, drainage depth and which step path ran
-    uint64_t tile_steps = 0, level_steps = 0;
+    uint64_t tile_steps = 0, level_steps = 0, halo16_steps = 0, halo32_steps = 0;
     int32_t nlevel = 0;
     fastscape::check(fs_path_stats(tm.handle(), &tile_steps, &level_steps), "fs_path_stats");
+    fastscape::check(fs_tile_stats(tm.handle(), &halo16_steps, &halo32_steps), "fs_tile_stats");
     fastscape::check(fs_get_levels(tm.handle(), nullptr, 0, &nlevel), "fs_get_levels");
     const double cell_steps = static_cast<double>(width) * height * (nstep + 1);
     std::cout << "m cell_steps_per_s = " << (us > 0 ? cell_steps / (us * 1e-6) : 0.0) << std::endl;
     std::cout << "m nlevel_last_step = " << nlevel << std::endl;
     std::cout << "m tile_path_steps = " << tile_steps << std::endl;
     std::cout << "m level_path_steps = " << level_steps << std::endl;
+    std::cout << "m tile_halo32_steps = " << halo32_steps << std::endl;
     fastscape::PrintDEM(output_name, tm.getH(), width, height);
   } catch (const fastscape::Error &e) {
     std::cerr << "error: " << e.what() << std::endl;
