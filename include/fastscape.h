@@ -127,8 +127,9 @@ int fs_set_h(fs_model *m, const double *in, size_t count);
  * times are device time between HIP events bracketing each stage; Overall is
  * host wall time of fs_run / fs_step plus construction.  On the tile path
  * (below) donors, levels and accumulation are one kernel, reported under
- * DetermineDonors; GenerateOrder is the eligibility read-back; uplift is
- * fused into Erosion.  FS_PATH_LEVELS gives the reference's stage split. */
+ * DetermineDonors; GenerateOrder, FlowAcc and Uplift are then empty (the
+ * erosion follows the accumulation directly and uplift is fused into
+ * Erosion).  FS_PATH_LEVELS gives the reference's stage split. */
 int fs_get_timers(fs_model *m, uint64_t us[FS_NUM_TIMERS]);
 int fs_reset_timers(fs_model *m);
 
