@@ -231,29 +231,47 @@ __device__ __forceinline__ void load_region_words_x4(const uint8_t *__restrict__
 }
 
 // Mark (bitmask over region cells) the halo cells on the receiver chains of
-// the tile's cells: walks start at tile cells whose receiver leaves the tile
-// and stop at an outlet, the region edge, a tile cell, an already marked cell
-// or after kHalo hops.  Each newly marked cell is also appended to list (the
+// the tile's cells: walks start at the tile's border cells and stop at an
+// outlet, the region edge, a tile cell, an already marked cell or after
+// kHalo hops.  Each newly marked cell is also appended to list (the
 // first kHaloCap of them; *nlist counts them all).  mark and *nlist must be
 // zero and w_s loaded before the call (counts may still be changing: only the
 // offset and the tile bit are read).
 template <int BLOCK>
 __device__ __forceinline__ void mark_halo_chains(const uint32_t *w_s, uint32_t *mark,
                                                  uint16_t *list, uint32_t *nlist) {
-  for (int it = 0; it < kTile * kTile / BLOCK; it++) {
-    const int j = it * BLOCK + threadIdx.x;
-    int c = (kHalo + j / kTile) * kReg + kHalo + j % kTile;
-    int code = acc_code(w_s[c]);
-    for (int hop = 0; hop <= kHalo && code > kStopMax; hop++) {
-      c += code;
-      const uint32_t wd = w_s[c];
-      if (wd & kAccTile) break;
-      const uint32_t bit = 1u << (c & 31);
-      if (atomicOr(&mark[c >> 5], bit) & bit) break;   // marked by another walk
-      const uint32_t k = atomicAdd(nlist, 1u);
-      if (k < static_cast<uint32_t>(kHaloCap)) list[k] = static_cast<uint16_t>(c);
-      code = acc_code(wd);
-    }
+  // A chain leaves the tile from a cell of its border and re-enters (if it
+  // does) at another border cell, so walks from the 4*kTile-4 border cells,
+  // one per thread, mark every halo cell on the chains of all tile cells.
+  constexpr int kBorder = 4 * kTile - 4;
+  static_assert(kBorder <= BLOCK, "one border cell per thread");
+  const int b = threadIdx.x;
+  if (b >= kBorder) return;
+  int tx, ty;
+  if (b < kTile) {
+    tx = b;
+    ty = 0;
+  } else if (b < 2 * kTile) {
+    tx = b - kTile;
+    ty = kTile - 1;
+  } else if (b < 3 * kTile - 2) {
+    tx = 0;
+    ty = b - 2 * kTile + 1;
+  } else {
+    tx = kTile - 1;
+    ty = b - (3 * kTile - 2) + 1;
+  }
+  int c = (kHalo + ty) * kReg + kHalo + tx;
+  int code = acc_code(w_s[c]);
+  for (int hop = 0; hop <= kHalo && code > kStopMax; hop++) {
+    c += code;
+    const uint32_t wd = w_s[c];
+    if (wd & kAccTile) break;
+    const uint32_t bit = 1u << (c & 31);
+    if (atomicOr(&mark[c >> 5], bit) & bit) break;   // marked by another walk
+    const uint32_t k = atomicAdd(nlist, 1u);
+    if (k < static_cast<uint32_t>(kHaloCap)) list[k] = static_cast<uint16_t>(c);
+    code = acc_code(wd);
   }
 }
 
