@@ -903,52 +903,50 @@ int fs_ensemble_create(const fs_realization *reals, int count, int device, fs_en
   FS_HIP_TRY(hipMemset(e->hist, 0, static_cast<size_t>(count) * kHistLen * sizeof(int32_t)));
   FS_HIP_TRY(hipMemset(e->ctl, 0, kCtlInts * sizeof(int32_t)));
 
-  // initial terrains and erosion factor tables, realizations spread over threads
+  // Initial terrains and erosion factor tables.  Host threads fill host
+  // buffers, one realization each per round; this thread alone makes every
+  // HIP call (no HIP state is created on threads that exit).
   unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   nt = std::min<unsigned>(nt, static_cast<unsigned>(count));
-  std::vector<int> status(nt, FS_OK);
-  std::vector<std::string> errs(nt);
-  std::vector<std::thread> pool;
-  for (unsigned w = 0; w < nt; w++) {
-    pool.emplace_back([&, w]() {
-      if (hipSetDevice(device) != hipSuccess) {
-        status[w] = FS_ERR_HIP;
+  std::vector<std::vector<double>> terr(nt), tabs(nt);
+  for (int r0 = 0; r0 < count; r0 += static_cast<int>(nt)) {
+    const int nr = std::min(static_cast<int>(nt), count - r0);
+    std::vector<int> status(static_cast<size_t>(nr), FS_OK);
+    auto fill = [&](int w) {
+      const int i = r0 + w;
+      const EnsReal &g = e->real[i];
+      const fs_realization &s = e->spec[i];
+      const size_t n = static_cast<size_t>(g.W) * g.H;
+      const size_t kmax = std::max<size_t>(n, kFactCache - 1);
+      try {
+        terr[w].resize(n);
+        tabs[w].resize(2 * (kmax + 1));
+      } catch (const std::bad_alloc &) {
+        status[w] = FS_ERR_OUT_OF_MEMORY;
         return;
       }
-      std::vector<double> buf;
-      for (int i = static_cast<int>(w); i < count; i += static_cast<int>(nt)) {
-        const EnsReal &g = e->real[i];
-        const fs_realization &s = e->spec[i];
-        const size_t n = static_cast<size_t>(g.W) * g.H;
-        const size_t kmax = std::max<size_t>(n, kFactCache - 1);
-        try {
-          buf.resize(std::max(n, 2 * (kmax + 1)));
-        } catch (const std::bad_alloc &) {
-          status[w] = FS_ERR_OUT_OF_MEMORY;
-          return;
-        }
-        fs::generate_terrain(buf.data(), g.W, g.H, s.engine_seed);
-        if (hipMemcpy(e->h[0] + g.off, buf.data(), n * sizeof(double), hipMemcpyHostToDevice) !=
-            hipSuccess) {
-          status[w] = FS_ERR_HIP;
-          return;
-        }
-        fs::fill_fact_table(buf.data(), kmax, s.params.cell_area, s.params.meq, g.ek.kdt,
-                            g.ek.lenpow0, g.ek.lenpow1);
-        if (hipMemcpy(e->ftab + g.ftab_off, buf.data(), 2 * (kmax + 1) * sizeof(double),
-                      hipMemcpyHostToDevice) != hipSuccess) {
-          status[w] = FS_ERR_HIP;
-          return;
-        }
-      }
-    });
-  }
-  for (auto &th : pool) th.join();
-  for (int s : status)
-    if (s != FS_OK) {
-      fs::set_last_error("initialising realizations failed");
-      return s;
+      fs::generate_terrain(terr[w].data(), g.W, g.H, s.engine_seed);
+      fs::fill_fact_table(tabs[w].data(), kmax, s.params.cell_area, s.params.meq, g.ek.kdt,
+                          g.ek.lenpow0, g.ek.lenpow1);
+    };
+    {
+      std::vector<std::thread> pool;
+      for (int w = 1; w < nr; w++) pool.emplace_back(fill, w);
+      fill(0);
+      for (auto &th : pool) th.join();
     }
+    for (int w = 0; w < nr; w++) {
+      if (status[w] != FS_OK) {
+        fs::set_last_error("initialising realizations failed");
+        return status[w];
+      }
+      const EnsReal &g = e->real[r0 + w];
+      FS_HIP_TRY(hipMemcpy(e->h[0] + g.off, terr[w].data(), terr[w].size() * sizeof(double),
+                           hipMemcpyHostToDevice));
+      FS_HIP_TRY(hipMemcpy(e->ftab + g.ftab_off, tabs[w].data(), tabs[w].size() * sizeof(double),
+                           hipMemcpyHostToDevice));
+    }
+  }
 
   int per_cu = 0;
   hipError_t oe;

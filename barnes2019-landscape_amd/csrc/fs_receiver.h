@@ -36,9 +36,11 @@ __host__ __device__ inline int receiver_reference(double hc, const double nb[8],
 // rounding, so that largest quotient is q = RN(A/dr) for the group's
 // largest drop A, and a smaller drop x can round to the same quotient only
 // when A - x <= dr*ulp(q) <= 2^-52*A; drops within 2^-50*A of A are divided
-// exactly, everything else is ruled out without a division.  Tiny A (where
-// q could be subnormal) and an overflowing q divide every drop.  *q_out = 0 when the group has
-// no positive slope.  NaN drops never compare greater, as in the reference.
+// exactly, everything else is ruled out without a division.  The window
+// needs q to be a normal number and A*2^-50 not to underflow: a subnormal or
+// overflowing q, or a tiny A, divides every drop.  *q_out = 0 when the group
+// has no positive slope.  NaN drops never compare greater, as in the
+// reference.
 __host__ __device__ inline int group_best(const double d[8], int first, double dr, double *q_out) {
   double A = -__builtin_huge_val();
   for (int n = first; n < 8; n += 2)
@@ -49,7 +51,8 @@ __host__ __device__ inline int group_best(const double d[8], int first, double d
   if (!(q > 0.0)) return kNoReceiver;
   *q_out = q;
   // subnormal or overflowing quotients: the relative window does not hold
-  const bool exact_all = A < 0x1p-900 || !(q < __builtin_huge_val());
+  const bool exact_all =
+      A < 0x1p-900 || !(q >= 0x1p-1022) || !(q < __builtin_huge_val());
   const double window = A - A * 0x1p-50;
   for (int n = first; n < 8; n += 2) {
     if (d[n] == A) return n;

@@ -177,7 +177,13 @@ int fs_get_accum(fs_model *m, double *accum, size_t count);
 /* ---- device access --------------------------------------------------------
  * Device pointer of the elevation field (W*H doubles on the model's device)
  * and the model's stream (a hipStream_t).  Work a caller enqueues on other
- * streams must be ordered against this stream by the caller. */
+ * streams must be ordered against this stream by the caller.
+ * Lifetime: the pointer names the current field only until the next
+ * fs_run / fs_step.  The tile path reads one buffer and writes the other
+ * (neighbouring tiles read each other's cells as halo), then exchanges
+ * them, so after a step the field may live in the other buffer: fetch the
+ * pointer again after every step.  Writes through a stale pointer are
+ * lost; reads through it see an older field. */
 int fs_device_h(fs_model *m, double **dptr);
 int fs_stream(fs_model *m, void **stream);
 int fs_synchronize(fs_model *m);

@@ -29,17 +29,18 @@ static inline double step_ulps(double x, int k) {
 int main(int argc, char **argv) {
   const long n_cases = argc > 1 ? std::atol(argv[1]) : 20000000;
   const double SQRT2 = 1.414213562373095048801688724209698078569671875376948;
-  // the last two lie outside the division-free decision's divisor range
-  const double dcs[6] = {1.0, 100.0, 30.0, 0.001, 1e-30, 1e30};
+  // the last four lie outside the division-free decision's divisor range;
+  // 2^123 and 2^200 make quotients of drops >= 2^-900 subnormal
+  const double dcs[8] = {1.0, 100.0, 30.0, 0.001, 1e-30, 1e30, 0x1.3p123, 0x1p200};
   const double specials[10] = {0.0, -0.0, std::numeric_limits<double>::infinity(),
                                -std::numeric_limits<double>::infinity(),
                                std::numeric_limits<double>::quiet_NaN(), 4.9e-324, 1e-310,
                                1e308, -1e308, 1.0};
   long bad = 0;
   for (long t = 0; t < n_cases; t++) {
-    const double dc = dcs[t % 6], dd = dc * SQRT2;
+    const double dc = dcs[t % 8], dd = dc * SQRT2;
     double hc, nb[8];
-    const int kind = static_cast<int>(rnd() % 6);
+    const int kind = static_cast<int>(rnd() % 7);
     if (kind == 0) {                         // uniform
       hc = u01();
       for (double &v : nb) v = u01();
@@ -60,10 +61,17 @@ int main(int argc, char **argv) {
     } else if (kind == 4) {                  // special values mixed in
       hc = (rnd() % 3 == 0) ? specials[rnd() % 10] : u01();
       for (double &v : nb) v = (rnd() % 3 == 0) ? specials[rnd() % 10] : u01();
-    } else {                                 // wide dynamic range
+    } else if (kind == 5) {                  // wide dynamic range
       const int e1 = static_cast<int>(rnd() % 2000) - 1000;
       hc = std::ldexp(u01() + 0.5, e1);
       for (double &v : nb) v = std::ldexp(u01() + 0.5, e1 - static_cast<int>(rnd() % 60));
+    } else {                                 // tiny near-tied drops (subnormal quotients)
+      const double drop = std::ldexp(u01() + 0.5, -860 - static_cast<int>(rnd() % 120));
+      hc = 0.0;
+      for (int n = 0; n < 8; n++) {
+        const double target = (n & 1) ? drop * SQRT2 : drop;
+        nb[n] = -step_ulps(target, static_cast<int>(rnd() % 9) - 4);
+      }
     }
     const int a = fs::receiver_reference(hc, nb, dc, dd);
     const int b = fs::receiver_fast(hc, nb, dc, dd);

@@ -399,3 +399,38 @@ def test_receivers_of_consecutive_tile_steps(oracle, W, H):
         h_in = m.getH().ravel().copy()
         m.step(1)
         np.testing.assert_array_equal(m.receivers(), oracle.receivers(h_in, W, H, op))
+
+
+def _hip_dtoh(ptr, count):
+    """hipMemcpy of `count` doubles from a device address (the HIP runtime's
+    own entry point, outside the library)."""
+    import ctypes
+    hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemcpy.restype = ctypes.c_int
+    out = np.empty(count, dtype=np.float64)
+    assert hip.hipMemcpy(out.ctypes.data, ptr, count * 8, 2) == 0   # device to host
+    return out
+
+
+def test_device_pointer_names_the_field_until_the_next_step():
+    """fs_device_h is valid until the next step: the tile path exchanges its
+    two elevation buffers, so the pointer is fetched again after each step
+    and then always holds what getH() returns."""
+    note("device pointer lifetime")
+    W, H = 300, 260
+    with fa.FastScape(W, H, gpu_params(1), engine_seed=3) as m:
+        p0 = m.device_h()
+        np.testing.assert_array_equal(_hip_dtoh(p0, W * H), m.getH().ravel())
+        m.step(1)
+        p1 = m.device_h()
+        np.testing.assert_array_equal(_hip_dtoh(p1, W * H), m.getH().ravel())
+        assert m.path_stats() == {"tile": 1, "levels": 0} and p1 != p0
+        m.step(1)
+        p2 = m.device_h()
+        assert p2 == p0
+        np.testing.assert_array_equal(_hip_dtoh(p2, W * H), m.getH().ravel())
+        m.set_path(fa.PATH_LEVELS)          # the level path updates in place
+        m.step(1)
+        assert m.device_h() == p2
+        np.testing.assert_array_equal(_hip_dtoh(p2, W * H), m.getH().ravel())
