@@ -37,6 +37,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -61,6 +63,12 @@ namespace tile32 {
 struct KTimed {
   int kid;
   hipEvent_t a, b;
+};
+
+// A run of levels swept by one launch.
+struct Span {
+  int lo, hi;   // inclusive level range
+  bool small;
 };
 
 struct fs_model {
@@ -109,6 +117,11 @@ struct fs_model {
   hipEvent_t ev[2][7] = {};
   bool ev_pending[2] = {false, false};
   int ev_cur = 0;
+
+  // false for a model whose device buffers are lent to it (the entry point
+  // that allocates nothing): no lazy allocations, no events, no side stream
+  bool owns_memory = true;
+  std::vector<Span> spans;  // level plan of the current sweep
 
   bool ktiming = false;
   std::vector<KTimed> kpending;
@@ -180,6 +193,12 @@ int stage_collect(fs_model *m, int set) {
 
 int check_launch() {
   FS_HIP_TRY(hipGetLastError());
+  return FS_OK;
+}
+
+// Record a stage event; a model without events (lent buffers) skips it.
+int mark(hipEvent_t e, hipStream_t s) {
+  if (e) FS_HIP_TRY(hipEventRecord(e, s));
   return FS_OK;
 }
 
@@ -334,16 +353,11 @@ int fetch_header(fs_model *m, int ncounts) {
   return FS_OK;
 }
 
-// Plan a level sweep: maximal runs of levels whose size is at most
+// Plan a level sweep into out: maximal runs of levels whose size is at most
 // kSmallLevel go to one single-workgroup launch; every larger level gets its
 // own grid-wide launch.
-struct Span {
-  int lo, hi;   // inclusive level range
-  bool small;
-};
-
-std::vector<Span> plan_levels(const std::vector<int32_t> &counts, int lo, int hi) {
-  std::vector<Span> out;
+void plan_levels(const std::vector<int32_t> &counts, int lo, int hi, std::vector<Span> &out) {
+  out.clear();
   int L = lo;
   while (L <= hi) {
     if (counts[L] > kSmallLevel) {
@@ -356,7 +370,6 @@ std::vector<Span> plan_levels(const std::vector<int32_t> &counts, int lo, int hi
       L = e + 1;
     }
   }
-  return out;
 }
 
 template <int NMODE>
@@ -467,7 +480,8 @@ int build_order(fs_model *m, hipEvent_t ev_topology_done) {
     m->ticket_base += tiles;
   }
   if (D >= 1) {
-    for (const Span &sp : plan_levels(m->counts, 0, D - 1)) {
+    plan_levels(m->counts, 0, D - 1, m->spans);
+    for (const Span &sp : m->spans) {
       if (sp.small) {
         FS_TRY(kt_begin(m, FS_K_BFS));
         hipLaunchKernelGGL(k_bfs_small, dim3(1), dim3(kSmallBlock), 0, s, m->stack, m->dmask,
@@ -499,11 +513,11 @@ int level_path(fs_model *m, hipEvent_t *ev) {
   const int32_t N = m->N;
   FS_TRY(build_order(m, ev[2]));
   const int D = static_cast<int>(m->counts.size()) - 1;
-  FS_HIP_TRY(hipEventRecord(ev[3], s));
+  FS_TRY(mark(ev[3], s));
   // ---- FlowAcc: levels D down to 0 ----
   {
-    std::vector<Span> spans = plan_levels(m->counts, 0, D);
-    for (auto it = spans.rbegin(); it != spans.rend(); ++it) {
+    plan_levels(m->counts, 0, D, m->spans);
+    for (auto it = m->spans.rbegin(); it != m->spans.rend(); ++it) {
       if (it->small) {
         FS_TRY(kt_begin(m, FS_K_FLOWACC));
         hipLaunchKernelGGL(k_flowacc_small, dim3(1), dim3(kSmallBlock), 0, s, m->stack, m->dmask,
@@ -520,7 +534,7 @@ int level_path(fs_model *m, hipEvent_t *ev) {
       }
     }
   }
-  FS_HIP_TRY(hipEventRecord(ev[4], s));
+  FS_TRY(mark(ev[4], s));
   // ---- Uplift of the interior outlets ----
   {
     const int32_t n0 = m->counts[0];
@@ -530,13 +544,13 @@ int level_path(fs_model *m, hipEvent_t *ev) {
     FS_TRY(check_launch());
     FS_TRY(kt_end(m));
   }
-  FS_HIP_TRY(hipEventRecord(ev[5], s));
+  FS_TRY(mark(ev[5], s));
   // ---- Erode: levels 1..D ----
   if (D >= 1) {
-    const std::vector<Span> spans = plan_levels(m->counts, 1, D);
-    if (m->nmode == 1) FS_TRY(launch_erosion<1>(m, spans));
-    else if (m->nmode == 2) FS_TRY(launch_erosion<2>(m, spans));
-    else FS_TRY(launch_erosion<0>(m, spans));
+    plan_levels(m->counts, 1, D, m->spans);
+    if (m->nmode == 1) FS_TRY(launch_erosion<1>(m, m->spans));
+    else if (m->nmode == 2) FS_TRY(launch_erosion<2>(m, m->spans));
+    else FS_TRY(launch_erosion<0>(m, m->spans));
   }
   m->level_steps++;
   return FS_OK;
@@ -576,26 +590,38 @@ int tile_attempt(fs_model *m, int halo, int tiles, int tiles_x, bool clear, hipE
     hipLaunchKernelGGL(k_tile_acc, dim3(tiles), dim3(kAccBlock), 0, s, m->rcv, m->cnt, m->lvl,
                        m->plan, m->hdr, m->W, m->H, tiles_x, tiles);
   } else {
-    if (!m->plan32) FS_TRY(fs::dev_alloc(&m->plan32, static_cast<size_t>(tiles)));
+    if (!m->plan32) {
+      if (!m->owns_memory) return FS_ERR_INTERNAL;   // lent buffers include plan32
+      FS_TRY(fs::dev_alloc(&m->plan32, static_cast<size_t>(tiles)));
+    }
     hipLaunchKernelGGL(tile32::k_tile_acc, dim3(tiles), dim3(tile32::kAccBlock), 0, s, m->rcv,
                        m->cnt, m->lvl, m->plan32, m->hdr, m->W, m->H, tiles_x, tiles);
   }
   FS_TRY(check_launch());
   FS_TRY(kt_end(m));
-  FS_HIP_TRY(hipEventRecord(ev[2], s));
-  FS_HIP_TRY(hipEventRecord(m->acc_done, s));
-  FS_HIP_TRY(hipStreamWaitEvent(m->side, m->acc_done, 0));
-  FS_HIP_TRY(hipMemcpyAsync(m->hdr_h, m->hdr, kHdrInts * sizeof(int32_t), hipMemcpyDeviceToHost,
-                            m->side));
-  FS_HIP_TRY(hipEventRecord(ev[3], s));
-  FS_HIP_TRY(hipEventRecord(ev[4], s));
-  FS_HIP_TRY(hipEventRecord(ev[5], s));
+  FS_TRY(mark(ev[2], s));
+  if (m->side) {
+    FS_HIP_TRY(hipEventRecord(m->acc_done, s));
+    FS_HIP_TRY(hipStreamWaitEvent(m->side, m->acc_done, 0));
+    FS_HIP_TRY(hipMemcpyAsync(m->hdr_h, m->hdr, kHdrInts * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, m->side));
+  }
+  FS_TRY(mark(ev[3], s));
+  FS_TRY(mark(ev[4], s));
+  FS_TRY(mark(ev[5], s));
   FS_TRY(kt_begin(m, FS_K_TILE_ERODE));
   if (m->nmode == 1) FS_TRY(launch_tile_erode<1>(m, tiles, tiles_x, halo));
   else if (m->nmode == 2) FS_TRY(launch_tile_erode<2>(m, tiles, tiles_x, halo));
   else FS_TRY(launch_tile_erode<0>(m, tiles, tiles_x, halo));
   FS_TRY(kt_end(m));
-  FS_HIP_TRY(hipStreamSynchronize(m->side));
+  if (m->side) {
+    FS_HIP_TRY(hipStreamSynchronize(m->side));
+  } else {
+    // no side stream: read the header behind the erosion on the model's stream
+    FS_HIP_TRY(hipMemcpyAsync(m->hdr_h, m->hdr, kHdrInts * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, s));
+    FS_HIP_TRY(hipStreamSynchronize(s));
+  }
   FS_TRY(device_error_status(m->hdr_h[2]));
   *fits = m->hdr_h[3] == 0;
   return FS_OK;
@@ -636,9 +662,9 @@ int tile_path(fs_model *m, hipEvent_t *ev, bool *done) {
 // One time-step.
 int step_once(fs_model *m) {
   hipEvent_t *ev = m->ev[m->ev_cur];
-  FS_HIP_TRY(hipEventRecord(ev[0], m->stream));
+  FS_TRY(mark(ev[0], m->stream));
   FS_TRY(launch_receivers(m));
-  FS_HIP_TRY(hipEventRecord(ev[1], m->stream));
+  FS_TRY(mark(ev[1], m->stream));
   bool done = false;
   if (m->path == FS_PATH_AUTO && m->tile_halo != 0) FS_TRY(tile_path(m, ev, &done));
   if (!done) {
@@ -647,8 +673,8 @@ int step_once(fs_model *m) {
     const int D = static_cast<int>(m->counts.size()) - 1;
     m->tile_halo = D <= kHalo ? kHalo : D <= tile32::kHalo ? tile32::kHalo : 0;
   }
-  FS_HIP_TRY(hipEventRecord(ev[6], m->stream));
-  m->ev_pending[m->ev_cur] = true;
+  FS_TRY(mark(ev[6], m->stream));
+  m->ev_pending[m->ev_cur] = ev[0] != nullptr;
   m->ev_cur ^= 1;
   m->have_step = true;
   return FS_OK;
@@ -955,3 +981,213 @@ const char *fs_kernel_name(int kernel) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// the entry point that allocates nothing (fs_step_noalloc)
+// ===========================================================================
+// Every work array of a time-step lives in one file-scope device array,
+// `scratch`, sized for grids of at most kNaMaxCells cells; each GPU holds its
+// own copy for the life of the process.  Host code never passes `scratch`
+// itself (in host code the name is a stand-in, not a device address): at
+// set-up, once per device with that device current, hipGetSymbolAddress
+// gives that device's address, and the work arrays are carved from it.
+namespace {
+
+constexpr size_t kNaMaxCells = FS_NOALLOC_MAX_CELLS;
+constexpr int kNaMaxTiles = 512;        // tile plans held; wider grids take the level path
+constexpr int kNaMaxDevices = 64;
+
+constexpr size_t na_align(size_t v) { return (v + 255) & ~size_t{255}; }
+
+// Byte offsets of the work arrays in `scratch` (each 256-byte aligned).
+struct NaLayout {
+  size_t h_alt, rcv, dmask, lvl, depth, jump, list, stack, cnt, ftab, hdr, levels, status,
+      ticket, plan, plan32, total;
+};
+
+constexpr NaLayout na_layout() {
+  NaLayout L{};
+  size_t o = 0;
+  const size_t n = kNaMaxCells;
+  L.h_alt = o;  o = na_align(o + 8 * n);
+  L.rcv = o;    o = na_align(o + n + 16);
+  L.dmask = o;  o = na_align(o + n);
+  L.lvl = o;    o = na_align(o + n);
+  L.depth = o;  o = na_align(o + 2 * n);
+  L.jump = o;   o = na_align(o + 8 * n);
+  L.list = o;   o = na_align(o + 4 * n);
+  L.stack = o;  o = na_align(o + 4 * n);
+  L.cnt = o;    o = na_align(o + 4 * n);
+  L.ftab = o;   o = na_align(o + 16 * ((n > kFactCache - 1 ? n : kFactCache - 1) + 1));
+  L.hdr = o;    o = na_align(o + 4 * static_cast<size_t>(kHdrAlloc));
+  L.levels = o; o = na_align(o + 4 * static_cast<size_t>(FS_MAX_LEVELS + 4));
+  L.status = o; o = na_align(o + 8 * (n / kBfsTile + 2));
+  L.ticket = o; o = na_align(o + 8);
+  L.plan = o;   o = na_align(o + sizeof(TilePlan) * kNaMaxTiles);
+  L.plan32 = o; o = na_align(o + sizeof(tile32::TilePlan) * kNaMaxTiles);
+  L.total = o;
+  return L;
+}
+constexpr NaLayout kNa = na_layout();
+static_assert(kNa.total <= (size_t{64} << 20), "scratch stays within 64 MiB");
+
+__device__ __align__(256) unsigned char scratch[kNa.total];
+
+// Vector-store fill of n bytes at a 16-byte aligned address with byte v
+// (memory a caller's graph may hold is cleared by a kernel, not a memset).
+__global__ __launch_bounds__(256) void k_fill_bytes(uint8_t *__restrict__ p, uint32_t v,
+                                                    size_t n) {
+  const uint32_t v4 = (v & 0xFFu) * 0x01010101u;
+  const size_t nv = n / 16;
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < nv; i += gridDim.x * 256ull)
+    reinterpret_cast<uint4 *>(p)[i] = make_uint4(v4, v4, v4, v4);
+  if (blockIdx.x == 0 && threadIdx.x < n % 16) p[nv * 16 + threadIdx.x] = static_cast<uint8_t>(v);
+}
+
+struct NaDevice {
+  std::mutex lock;               // calls on one device run one at a time
+  bool ready = false;
+  unsigned char *base = nullptr; // this device's address of `scratch`
+  fs_model *m = nullptr;         // model shell over the carved arrays
+  std::vector<int32_t> hdr_host, levels_host;
+  std::unique_ptr<double[]> tab_host;
+  // parameters the factor table in `scratch` was filled for, and its size
+  bool tab_valid = false;
+  double tab_key[5] = {};
+  size_t tab_kmax = 0;
+};
+NaDevice g_na[kNaMaxDevices];
+
+// Once per device, with that device current.
+int na_setup(int dev, NaDevice &d) {
+  void *ptr = nullptr;
+  FS_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return FS_ERR_INTERNAL;
+  unsigned char *b = static_cast<unsigned char *>(ptr);
+  fs_model *m = new (std::nothrow) fs_model();
+  if (!m) return FS_ERR_OUT_OF_MEMORY;
+  try {
+    m->counts.reserve(FS_MAX_LEVELS + 4);
+    m->levels.reserve(FS_MAX_LEVELS + 4);
+    m->spans.reserve(FS_MAX_LEVELS + 4);
+    d.hdr_host.assign(kHdrInts + FS_MAX_LEVELS + 3, 0);
+    d.levels_host.assign(FS_MAX_LEVELS + 4, 0);
+    d.tab_host.reset(new double[2 * (kNaMaxCells + 1)]);
+  } catch (const std::bad_alloc &) {
+    delete m;
+    return FS_ERR_OUT_OF_MEMORY;
+  }
+  m->owns_memory = false;
+  m->device = dev;
+  m->h_alt = reinterpret_cast<double *>(b + kNa.h_alt);
+  m->rcv = b + kNa.rcv;
+  m->dmask = b + kNa.dmask;
+  m->lvl = b + kNa.lvl;
+  m->depth = reinterpret_cast<uint16_t *>(b + kNa.depth);
+  m->jump = reinterpret_cast<unsigned long long *>(b + kNa.jump);
+  m->list = reinterpret_cast<int32_t *>(b + kNa.list);
+  m->stack = reinterpret_cast<int32_t *>(b + kNa.stack);
+  m->cnt = reinterpret_cast<uint32_t *>(b + kNa.cnt);
+  m->ftab = reinterpret_cast<double *>(b + kNa.ftab);
+  m->hdr = reinterpret_cast<int32_t *>(b + kNa.hdr);
+  m->levels_d = reinterpret_cast<int32_t *>(b + kNa.levels);
+  m->status = reinterpret_cast<uint64_t *>(b + kNa.status);
+  m->ticket = reinterpret_cast<unsigned long long *>(b + kNa.ticket);
+  m->plan = reinterpret_cast<TilePlan *>(b + kNa.plan);
+  m->plan32 = reinterpret_cast<tile32::TilePlan *>(b + kNa.plan32);
+  m->hdr_h = d.hdr_host.data();
+  m->levels_h = d.levels_host.data();
+  d.base = b;
+  d.m = m;
+  d.ready = true;
+  return FS_OK;
+}
+
+int na_fill(hipStream_t s, void *p, uint32_t v, size_t n) {
+  const unsigned blocks = static_cast<unsigned>(std::min<size_t>((n / 16 + 255) / 256 + 1, 1024));
+  hipLaunchKernelGGL(k_fill_bytes, dim3(blocks), dim3(256), 0, s, static_cast<uint8_t *>(p), v, n);
+  return check_launch();
+}
+
+}  // namespace
+
+extern "C" int fs_step_noalloc(double *h, int width, int height, const fs_params *params,
+                               int iterations, void *stream) {
+  if (!h || !params || iterations < 0) return FS_ERR_INVALID_ARGUMENT;
+  FS_TRY(fs::validate_params(width, height, params));
+  const size_t N = static_cast<size_t>(width) * static_cast<size_t>(height);
+  if (N > kNaMaxCells) {
+    fs::set_last_error("fs_step_noalloc holds grids of at most 1048576 cells");
+    return FS_ERR_DIMENSIONS;
+  }
+  int dev = 0;
+  FS_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNaMaxDevices) return FS_ERR_NO_DEVICE;
+  NaDevice &d = g_na[dev];
+  std::lock_guard<std::mutex> guard(d.lock);
+  if (!d.ready) FS_TRY(na_setup(dev, d));
+  fs_model *m = d.m;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int tiles = ((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
+  // the call's grid and parameters; no step taken yet
+  m->W = width;
+  m->H = height;
+  m->N = static_cast<int32_t>(N);
+  m->p = *params;
+  m->d_card = params->dx;
+  m->d_diag = params->dx * fs::kSqrt2;
+  m->ek = make_erode_const(*params, &m->nmode);
+  m->stream = s;
+  m->h = h;
+  m->h_alt = reinterpret_cast<double *>(d.base + kNa.h_alt);
+  m->have_step = m->order_current = false;
+  m->path = tiles <= kNaMaxTiles ? FS_PATH_AUTO : FS_PATH_LEVELS;
+  m->tile_halo = kHalo;
+  m->prev_maxdepth = FS_MAX_LEVELS;
+  m->ticket_base = 0;
+  m->epoch = 0;
+  m->nlevel = 0;
+  m->tile_steps = m->level_steps = m->tile32_steps = 0;
+  // state that must start defined: every receiver NO_FLOW (rings never get
+  // one), counts, the header, the scan status words and the ticket
+  const size_t ntiles = N / kBfsTile + 2;
+  FS_TRY(na_fill(s, m->rcv, 0xFF, N + 16));
+  FS_TRY(na_fill(s, m->cnt, 0, 4 * N));
+  FS_TRY(na_fill(s, m->hdr, 0, 4 * static_cast<size_t>(kHdrAlloc)));
+  FS_TRY(na_fill(s, m->status, 0, 8 * ntiles));
+  FS_TRY(na_fill(s, m->ticket, 0, 8));
+  // erosion factors for counts 0..max(N, kFactCache - 1), kept across calls
+  // with the same parameters
+  const size_t kmax = std::max<size_t>(N, kFactCache - 1);
+  const double key[5] = {params->cell_area, params->meq, m->ek.kdt, m->ek.lenpow0,
+                         m->ek.lenpow1};
+  if (!d.tab_valid || d.tab_kmax < kmax || std::memcmp(key, d.tab_key, sizeof(key)) != 0) {
+    d.tab_valid = false;
+    fs::fill_fact_table(d.tab_host.get(), kmax, params->cell_area, params->meq, m->ek.kdt,
+                        m->ek.lenpow0, m->ek.lenpow1, 1);
+    FS_HIP_TRY(hipMemcpyAsync(m->ftab, d.tab_host.get(), 2 * (kmax + 1) * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+    FS_HIP_TRY(hipStreamSynchronize(s));
+    std::memcpy(d.tab_key, key, sizeof(key));
+    d.tab_kmax = kmax;
+    d.tab_valid = true;
+  }
+  int st = run_steps(m, iterations);
+  if (st == FS_OK && m->h != h) {
+    // an odd number of tile steps left the field in the second buffer
+    const hipError_t e = hipMemcpyAsync(h, m->h, N * sizeof(double), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) {
+      const hipError_t e2 = hipStreamSynchronize(s);
+      if (e2 != hipSuccess) {
+        fs::set_last_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e2));
+        st = FS_ERR_HIP;
+      }
+    } else {
+      fs::set_last_error(std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+      st = FS_ERR_HIP;
+    }
+  }
+  m->h = nullptr;
+  m->stream = nullptr;
+  return st;
+}

@@ -89,25 +89,31 @@ void generate_terrain(double *h, int W, int H, uint32_t engine_seed) {
 // upstream cell count k, so tab[2k] (cardinal receiver, dr = dx) and
 // tab[2k+1] (diagonal, dr = dx*SQRT2) hold the factor evaluated with the host
 // libm and the reference's order of operations, ((keq*dt)*A^m)/dr^n, for
-// k = 0..kmax.
+// k = 0..kmax.  Uses at most max_threads host threads; with one, the
+// calling thread fills the table and no thread is started.
 void fill_fact_table(double *tab, size_t kmax, double cell_area, double meq, double kdt,
-                     double lenpow_card, double lenpow_diag) {
+                     double lenpow_card, double lenpow_diag, unsigned max_threads) {
   unsigned nt = std::thread::hardware_concurrency();
-  nt = std::max(1u, std::min(nt, 16u));
+  nt = std::max(1u, std::min({nt, 16u, std::max(1u, max_threads)}));
   const size_t n = kmax + 1;
   if (n < (1u << 16)) nt = 1;
+  auto fill = [=](size_t a, size_t b) {
+    for (size_t k = a; k < b; k++) {
+      const double am = std::pow(static_cast<double>(k) * cell_area, meq);
+      tab[2 * k] = (kdt * am) / lenpow_card;
+      tab[2 * k + 1] = (kdt * am) / lenpow_diag;
+    }
+  };
+  if (nt == 1) {
+    fill(0, n);
+    return;
+  }
   std::vector<std::thread> pool;
   const size_t chunk = (n + nt - 1) / nt;
   for (unsigned t = 0; t < nt; t++) {
     const size_t a = t * chunk, b = std::min(n, a + chunk);
     if (a >= b) break;
-    pool.emplace_back([=]() {
-      for (size_t k = a; k < b; k++) {
-        const double am = std::pow(static_cast<double>(k) * cell_area, meq);
-        tab[2 * k] = (kdt * am) / lenpow_card;
-        tab[2 * k + 1] = (kdt * am) / lenpow_diag;
-      }
-    });
+    pool.emplace_back(fill, a, b);
   }
   for (auto &th : pool) th.join();
 }

@@ -17,7 +17,7 @@ const char *last_error();
 int validate_params(int W, int H, const fs_params *p);
 void generate_terrain(double *h, int W, int H, uint32_t engine_seed);
 void fill_fact_table(double *tab, size_t kmax, double cell_area, double meq, double kdt,
-                     double lenpow_card, double lenpow_diag);
+                     double lenpow_card, double lenpow_diag, unsigned max_threads = 16);
 int write_dem(const char *filename, const double *h, int W, int H);
 const char *status_string(int s);
 

@@ -31,6 +31,7 @@ TILE_HALO_DEEP = 32
 # Every symbol declared in include/fastscape.h.
 EXPORTED = [
     "fs_params_preset", "fs_create", "fs_create_from_h", "fs_destroy", "fs_run", "fs_step",
+    "fs_step_noalloc",
     "fs_get_h", "fs_set_h", "fs_get_timers", "fs_reset_timers", "fs_get_receivers",
     "fs_get_donors", "fs_get_stack", "fs_get_levels", "fs_get_accum", "fs_device_h",
     "fs_stream", "fs_synchronize", "fs_set_step_path", "fs_path_stats", "fs_tile_stats", "fs_set_kernel_timing", "fs_kernel_stats",
@@ -105,6 +106,8 @@ def _lib():
         "fs_destroy": (ctypes.c_int, [_vp]),
         "fs_run": (ctypes.c_int, [_vp, ctypes.c_int]),
         "fs_step": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "fs_step_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(FsParams), ctypes.c_int, _vp]),
         "fs_get_h": (ctypes.c_int, [_vp, _dp, ctypes.c_size_t]),
         "fs_set_h": (ctypes.c_int, [_vp, _dp, ctypes.c_size_t]),
         "fs_get_timers": (ctypes.c_int, [_vp, _u64p]),

@@ -115,6 +115,24 @@ int fs_run(fs_model *m, int nstep);
  * by benchmarks and per-step parity checks. */
 int fs_step(fs_model *m, int iterations);
 
+/* The time-step of fs_step on a caller's elevation field, allocating
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device or pinned memory (serving loops, harnesses that forbid
+ * hipMalloc).  h: width*height doubles on the current HIP device, updated in
+ * place by `iterations` time-steps on `stream` (a hipStream_t; NULL = the
+ * null stream); same results, bit for bit, as fs_create_from_h + fs_step.
+ * Every work array comes from one device array built into the library,
+ * sized for at most FS_NOALLOC_MAX_CELLS cells (about 62 MB, held on each
+ * GPU the library runs on until the process ends); larger grids return
+ * FS_ERR_DIMENSIONS.  Grids of more than 512 64x64 tiles take the level
+ * path.  All calls on one GPU share that array, which keeps the last call's
+ * contents: calls on one device are serialised by a lock (overlapping calls
+ * would race on the shared array), and each returns after its device work
+ * has finished.  Stage timers and per-stage readouts are not kept. */
+#define FS_NOALLOC_MAX_CELLS 1048576
+int fs_step_noalloc(double *h, int width, int height, const fs_params *params, int iterations,
+                    void *stream);
+
 /* Copy the elevations (W*H doubles) device -> host.  Replaces getH()
  * (RB:403-405), which returns the reference's internal vector. */
 int fs_get_h(fs_model *m, double *out, size_t count);
