@@ -21,6 +21,12 @@
 #define GIT_HASH "unknown"
 #endif
 
+// Algorithmic HBM bytes of one cell-step: receivers 9 (h 8 + direction 1),
+// donors 2, order 10, accumulation 21, uplift + erosion 37; and MI355X's
+// HBM3E peak.
+constexpr double kBytesPerCellStep = 79.0;
+constexpr double kHbmPeakBytesPerS = 8.0e12;
+
 int main(int argc, char **argv) {
   int preset = FS_PRESET_REFERENCE, device = 0;
   std::vector<char *> args{argv[0]};
@@ -67,9 +73,20 @@ int main(int argc, char **argv) {
     fastscape::check(fs_path_stats(tm.handle(), &tile_steps, &level_steps), "fs_path_stats");
     fastscape::check(fs_tile_stats(tm.handle(), &halo16_steps, &halo32_steps), "fs_tile_stats");
     fastscape::check(fs_get_levels(tm.handle(), nullptr, 0, &nlevel), "fs_get_levels");
+    uint64_t t[FS_NUM_TIMERS];
+    fastscape::check(fs_get_timers(tm.handle(), t), "fs_get_timers");
     const double cell_steps = static_cast<double>(width) * height * (nstep + 1);
+    const double run_us = static_cast<double>(t[FS_T_OVERALL] - t[FS_T_INITIALIZE]);
+    const double step_rate = run_us > 0 ? cell_steps / (run_us * 1e-6) : 0.0;
     std::cout << "m cell_steps_per_s = " << (us > 0 ? cell_steps / (us * 1e-6) : 0.0) << std::endl;
+    // the time-steps alone (Overall minus Initialize), against the HBM
+    // roofline of the time-step's algorithmic bytes per cell-step
+    std::cout << "m step_cell_steps_per_s = " << step_rate << std::endl;
+    std::cout << "m roofline_bytes_per_cell_step = " << kBytesPerCellStep << std::endl;
+    std::cout << "m hbm_roofline_frac = " << step_rate * kBytesPerCellStep / kHbmPeakBytesPerS
+              << std::endl;
     std::cout << "m nlevel_last_step = " << nlevel << std::endl;
+    std::cout << "m deepest_level_last_step = " << (nlevel >= 3 ? nlevel - 3 : 0) << std::endl;
     std::cout << "m tile_path_steps = " << tile_steps << std::endl;
     std::cout << "m level_path_steps = " << level_steps << std::endl;
     std::cout << "m tile_halo32_steps = " << halo32_steps << std::endl;

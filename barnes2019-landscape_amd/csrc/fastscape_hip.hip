@@ -90,11 +90,20 @@ struct fs_model {
   unsigned long long *jump = nullptr;
   int32_t *list = nullptr, *stack = nullptr, *hdr = nullptr, *levels_d = nullptr;
   uint32_t *cnt = nullptr;
+  // the level path's records in stack order (index = position in `stack`)
+  uint8_t *rdir = nullptr;   // receiver direction
+  int32_t *fdon = nullptr;   // position of the first donor
+  uint8_t *ndon = nullptr;   // number of donors
+  uint32_t *cnt_s = nullptr; // upstream cell count
   double *ftab = nullptr;   // erosion factor per (upstream count, diagonal)
   TilePlan *plan = nullptr; // per-tile erosion plan of the tile path
   tile32::TilePlan *plan32 = nullptr;  // the same for the 32-cell halo (on first use)
   uint64_t *status = nullptr;
   unsigned long long *ticket = nullptr;
+  unsigned long long *phase = nullptr;   // tile kernels' phase ticks (stage timers)
+  int32_t *fill_flag = nullptr;          // a fill launch changed a cell
+  long long steps_in_call = 0;           // steps taken by the current fs_run / fs_step
+  bool fill_mode = false;                // fill depressions before every step
 
   // host mirrors
   int32_t *hdr_h = nullptr;      // pinned: header + level counts
@@ -103,6 +112,7 @@ struct fs_model {
   int32_t nlevel = 0, maxdepth = 0, prev_maxdepth = FS_MAX_LEVELS;
   bool have_step = false;
   bool order_current = false;      // stack and donor masks match rcv
+  bool cnt_stack = false;          // the last step's counts are in cnt_s (level path)
   int path = FS_PATH_AUTO;
   uint64_t tile_steps = 0, level_steps = 0, tile32_steps = 0;
   // halo the next step tries first: 16 or 32; 0 = level path (the last
@@ -116,7 +126,9 @@ struct fs_model {
   double stage_ms[FS_NUM_TIMERS] = {};
   hipEvent_t ev[2][7] = {};
   bool ev_pending[2] = {false, false};
+  bool ev_tile[2] = {false, false};   // that step took the tile path
   int ev_cur = 0;
+  double tile_acc_ms = 0, tile_ero_ms = 0;   // tile kernels' time not yet split
 
   // false for a model whose device buffers are lent to it (the entry point
   // that allocates nothing): no lazy allocations, no events, no side stream
@@ -182,12 +194,58 @@ int kt_collect(fs_model *m) {
 
 int stage_collect(fs_model *m, int set) {
   if (!m->ev_pending[set]) return FS_OK;
-  for (int i = 0; i < 6; i++) {
-    float ms = 0.f;
-    FS_HIP_TRY(hipEventElapsedTime(&ms, m->ev[set][i], m->ev[set][i + 1]));
-    m->stage_ms[FS_T_RECEIVERS + i] += ms;
+  float ms[6] = {};
+  for (int i = 0; i < 6; i++)
+    FS_HIP_TRY(hipEventElapsedTime(&ms[i], m->ev[set][i], m->ev[set][i + 1]));
+  if (m->ev_tile[set]) {
+    // donors..erosion ran fused in the two tile kernels; split_tile_time
+    // assigns their time to the stages
+    m->stage_ms[FS_T_RECEIVERS] += ms[0];
+    m->tile_acc_ms += ms[1] + ms[2] + ms[3] + ms[4];
+    m->tile_ero_ms += ms[5];
+  } else {
+    for (int i = 0; i < 6; i++) m->stage_ms[FS_T_RECEIVERS + i] += ms[i];
   }
   m->ev_pending[set] = false;
+  return FS_OK;
+}
+
+// Split the tile kernels' event time over the reference's stages in
+// proportion to the phase ticks their sampled workgroups recorded on the
+// timed steps:
+// k_tile_acc's region load -> DetermineDonors, its chain walks and
+// count/level stores -> FlowAcc, its halo-chain marking and erosion plan ->
+// GenerateOrder; k_tile_erode's level bucketing -> GenerateOrder, its outlet
+// uplift -> Uplift, its wavefront and stores -> Erosion.
+int split_tile_time(fs_model *m) {
+  if (m->tile_acc_ms <= 0 && m->tile_ero_ms <= 0) return FS_OK;
+  double ph[kPhaseStride] = {};
+  if (m->phase) {
+    std::vector<unsigned long long> raw(static_cast<size_t>(kPhaseSlots) * kPhaseStride);
+    FS_HIP_TRY(hipMemcpy(raw.data(), m->phase, raw.size() * sizeof(raw[0]),
+                         hipMemcpyDeviceToHost));
+    for (int r = 0; r < kPhaseSlots; r++)
+      for (int i = 0; i < kPhaseStride; i++)
+        ph[i] += static_cast<double>(raw[static_cast<size_t>(r) * kPhaseStride + i]);
+    FS_HIP_TRY(hipMemset(m->phase, 0, raw.size() * sizeof(raw[0])));
+  }
+  const double acc = ph[kPhAccLoad] + ph[kPhAccWalk] + ph[kPhAccPlan] + ph[kPhAccStore];
+  if (acc > 0) {
+    m->stage_ms[FS_T_DONORS] += m->tile_acc_ms * ph[kPhAccLoad] / acc;
+    m->stage_ms[FS_T_FLOWACC] += m->tile_acc_ms * (ph[kPhAccWalk] + ph[kPhAccStore]) / acc;
+    m->stage_ms[FS_T_ORDER] += m->tile_acc_ms * ph[kPhAccPlan] / acc;
+  } else {
+    m->stage_ms[FS_T_FLOWACC] += m->tile_acc_ms;
+  }
+  const double ero = ph[kPhEroBucket] + ph[kPhEroRoots] + ph[kPhEroLoop] + ph[kPhEroStore];
+  if (ero > 0) {
+    m->stage_ms[FS_T_ORDER] += m->tile_ero_ms * ph[kPhEroBucket] / ero;
+    m->stage_ms[FS_T_UPLIFT] += m->tile_ero_ms * ph[kPhEroRoots] / ero;
+    m->stage_ms[FS_T_EROSION] += m->tile_ero_ms * (ph[kPhEroLoop] + ph[kPhEroStore]) / ero;
+  } else {
+    m->stage_ms[FS_T_EROSION] += m->tile_ero_ms;
+  }
+  m->tile_acc_ms = m->tile_ero_ms = 0;
   return FS_OK;
 }
 
@@ -222,10 +280,13 @@ void free_model(fs_model *m) {
                     static_cast<void *>(m->depth),
                     static_cast<void *>(m->jump), static_cast<void *>(m->list),
                     static_cast<void *>(m->stack), static_cast<void *>(m->hdr),
+                    static_cast<void *>(m->rdir), static_cast<void *>(m->fdon),
+                    static_cast<void *>(m->ndon), static_cast<void *>(m->cnt_s),
                     static_cast<void *>(m->levels_d), static_cast<void *>(m->cnt),
                     static_cast<void *>(m->ftab), static_cast<void *>(m->status),
                     static_cast<void *>(m->ticket), static_cast<void *>(m->plan),
-                    static_cast<void *>(m->plan32)})
+                    static_cast<void *>(m->plan32), static_cast<void *>(m->phase),
+                    static_cast<void *>(m->fill_flag)})
     if (ptr) (void)hipFree(ptr);
   if (m->hdr_h) (void)hipHostFree(m->hdr_h);
   if (m->levels_h) (void)hipHostFree(m->levels_h);
@@ -287,6 +348,10 @@ int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_
   FS_TRY(fs::dev_alloc(&m->list, N));
   FS_TRY(fs::dev_alloc(&m->stack, N));
   FS_TRY(fs::dev_alloc(&m->cnt, N));
+  FS_TRY(fs::dev_alloc(&m->rdir, N));
+  FS_TRY(fs::dev_alloc(&m->fdon, N));
+  FS_TRY(fs::dev_alloc(&m->ndon, N));
+  FS_TRY(fs::dev_alloc(&m->cnt_s, N));
   // at least kFactCache counts, so the tile kernels' LDS cache stays in bounds
   const size_t kmax = std::max<size_t>(N, kFactCache - 1);
   FS_TRY(fs::dev_alloc(&m->ftab, 2 * (kmax + 1)));
@@ -297,6 +362,8 @@ int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_
   const size_t plan_tiles = static_cast<size_t>((m->W + kTile - 1) / kTile) *
                             static_cast<size_t>((m->H + kTile - 1) / kTile);
   FS_TRY(fs::dev_alloc(&m->plan, plan_tiles));
+  FS_TRY(fs::dev_alloc(&m->phase, static_cast<size_t>(kPhaseSlots) * kPhaseStride));
+  FS_TRY(fs::dev_alloc(&m->fill_flag, 1));
   if (hipHostMalloc(reinterpret_cast<void **>(&m->hdr_h),
                     (kHdrInts + FS_MAX_LEVELS + 3) * sizeof(int32_t)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void **>(&m->levels_h),
@@ -309,6 +376,9 @@ int create_model(int W, int H, const fs_params *p, const double *h_init, uint32_
   FS_HIP_TRY(hipMemsetAsync(m->status, 0, ntiles * sizeof(uint64_t), m->stream));
   FS_HIP_TRY(hipMemsetAsync(m->ticket, 0, sizeof(unsigned long long), m->stream));
   FS_HIP_TRY(hipMemsetAsync(m->plan, 0, plan_tiles * sizeof(TilePlan), m->stream));
+  FS_HIP_TRY(hipMemsetAsync(m->phase, 0,
+                            static_cast<size_t>(kPhaseSlots) * kPhaseStride * sizeof(uint64_t),
+                            m->stream));
   FS_HIP_TRY(hipMemsetAsync(m->cnt, 0, N * sizeof(uint32_t), m->stream));
   {
     std::vector<double> tab;
@@ -379,7 +449,7 @@ int launch_erosion(fs_model *m, const std::vector<Span> &spans) {
     if (s.small) {
       FS_TRY(kt_begin(m, FS_K_EROSION));
       hipLaunchKernelGGL(k_erode_small<NMODE>, dim3(1), dim3(kSmallBlock), 0, m->stream, m->stack,
-                         m->rcv, m->cnt, m->ftab, m->h, m->levels_d, s.lo, s.hi, m->W, m->N,
+                         m->rdir, m->cnt_s, m->ftab, m->h, m->levels_d, s.lo, s.hi, m->W, m->N,
                          m->ek, hdr);
       FS_TRY(check_launch());
       FS_TRY(kt_end(m));
@@ -387,8 +457,8 @@ int launch_erosion(fs_model *m, const std::vector<Span> &spans) {
       const int32_t ls = m->levels[s.lo], le = m->levels[s.lo + 1];
       const unsigned blocks = static_cast<unsigned>((le - ls + 255) / 256);
       FS_TRY(kt_begin(m, FS_K_EROSION));
-      hipLaunchKernelGGL(k_erode<NMODE>, dim3(blocks), dim3(256), 0, m->stream, m->stack, m->rcv,
-                         m->cnt, m->ftab, m->h, ls, le, m->W, m->N, m->ek, hdr);
+      hipLaunchKernelGGL(k_erode<NMODE>, dim3(blocks), dim3(256), 0, m->stream, m->stack,
+                         m->rdir, m->cnt_s, m->ftab, m->h, ls, le, m->W, m->N, m->ek, hdr);
       FS_TRY(check_launch());
       FS_TRY(kt_end(m));
     }
@@ -485,15 +555,17 @@ int build_order(fs_model *m, hipEvent_t ev_topology_done) {
       if (sp.small) {
         FS_TRY(kt_begin(m, FS_K_BFS));
         hipLaunchKernelGGL(k_bfs_small, dim3(1), dim3(kSmallBlock), 0, s, m->stack, m->dmask,
-                           m->levels_d, sp.lo, sp.hi + 1, W, N, m->hdr);
+                           m->rdir, m->fdon, m->ndon, m->levels_d, sp.lo, sp.hi + 1, W, N,
+                           m->hdr);
         FS_TRY(check_launch());
         FS_TRY(kt_end(m));
       } else {
         const int32_t ls = m->levels[sp.lo], le = m->levels[sp.lo + 1];
         const unsigned tiles = static_cast<unsigned>((le - ls + kBfsTile - 1) / kBfsTile);
         FS_TRY(kt_begin(m, FS_K_BFS));
-        hipLaunchKernelGGL(k_bfs, dim3(tiles), dim3(kScanBlock), 0, s, m->stack, m->dmask, ls, le,
-                           W, N, m->status, m->ticket, m->ticket_base, next_epoch(m), m->hdr);
+        hipLaunchKernelGGL(k_bfs, dim3(tiles), dim3(kScanBlock), 0, s, m->stack, m->dmask,
+                           m->rdir, m->fdon, m->ndon, ls, le, W, N, m->status, m->ticket,
+                           m->ticket_base, next_epoch(m), m->hdr);
         FS_TRY(check_launch());
         FS_TRY(kt_end(m));
         m->ticket_base += tiles;
@@ -520,15 +592,15 @@ int level_path(fs_model *m, hipEvent_t *ev) {
     for (auto it = m->spans.rbegin(); it != m->spans.rend(); ++it) {
       if (it->small) {
         FS_TRY(kt_begin(m, FS_K_FLOWACC));
-        hipLaunchKernelGGL(k_flowacc_small, dim3(1), dim3(kSmallBlock), 0, s, m->stack, m->dmask,
-                           m->cnt, m->levels_d, it->hi, it->lo, W, N, m->hdr);
+        hipLaunchKernelGGL(k_flowacc_small, dim3(1), dim3(kSmallBlock), 0, s, m->fdon, m->ndon,
+                           m->cnt_s, m->levels_d, it->hi, it->lo, D, N, m->hdr);
         FS_TRY(check_launch());
         FS_TRY(kt_end(m));
       } else {
         const int32_t ls = m->levels[it->lo], le = m->levels[it->lo + 1];
         FS_TRY(kt_begin(m, FS_K_FLOWACC));
         hipLaunchKernelGGL(k_flowacc, dim3(static_cast<unsigned>((le - ls + 255) / 256)), dim3(256),
-                           0, s, m->stack, m->dmask, m->cnt, ls, le, W, N, m->hdr);
+                           0, s, m->fdon, m->ndon, m->cnt_s, ls, le, it->lo == D, N, m->hdr);
         FS_TRY(check_launch());
         FS_TRY(kt_end(m));
       }
@@ -553,20 +625,27 @@ int level_path(fs_model *m, hipEvent_t *ev) {
     else FS_TRY(launch_erosion<0>(m, m->spans));
   }
   m->level_steps++;
+  m->cnt_stack = true;
   return FS_OK;
 }
 
-template <int NMODE>
+template <int NMODE, bool kTimed>
 int launch_tile_erode(fs_model *m, int tiles, int tiles_x, int halo) {
   if (halo == kHalo)
-    hipLaunchKernelGGL(k_tile_erode<NMODE>, dim3(tiles), dim3(kEroBlock), 0, m->stream, m->h,
-                       m->h_alt, m->plan, m->lvl, m->cnt, m->ftab, m->W, m->H, tiles_x, tiles,
-                       m->ek, m->hdr);
+    hipLaunchKernelGGL((k_tile_erode<NMODE, kTimed>), dim3(tiles), dim3(kEroBlock), 0, m->stream,
+                       m->h, m->h_alt, m->plan, m->lvl, m->cnt, m->ftab, m->W, m->H, tiles_x,
+                       tiles, m->ek, m->hdr, m->phase);
   else
-    hipLaunchKernelGGL(tile32::k_tile_erode<NMODE>, dim3(tiles), dim3(tile32::kEroBlock), 0,
-                       m->stream, m->h, m->h_alt, m->plan32, m->lvl, m->cnt, m->ftab, m->W, m->H,
-                       tiles_x, tiles, m->ek, m->hdr);
+    hipLaunchKernelGGL((tile32::k_tile_erode<NMODE, kTimed>), dim3(tiles), dim3(tile32::kEroBlock),
+                       0, m->stream, m->h, m->h_alt, m->plan32, m->lvl, m->cnt, m->ftab, m->W,
+                       m->H, tiles_x, tiles, m->ek, m->hdr, m->phase);
   return check_launch();
+}
+
+template <int NMODE>
+int launch_tile_erode(fs_model *m, int tiles, int tiles_x, int halo, bool timed) {
+  return timed ? launch_tile_erode<NMODE, true>(m, tiles, tiles_x, halo)
+               : launch_tile_erode<NMODE, false>(m, tiles, tiles_x, halo);
 }
 
 // Accumulation with the given halo (16 or 32), then the erosion with the
@@ -585,17 +664,30 @@ int tile_attempt(fs_model *m, int halo, int tiles, int tiles_x, bool clear, hipE
     hipLaunchKernelGGL(k_fill_counts, dim3(1), dim3(64), 0, s, m->hdr, 0);
     FS_TRY(check_launch());
   }
+  // one step in kPhaseEvery (and the first of each call) runs the timed
+  // instantiations that sample the phase ticks of the stage timers
+  const bool timed = m->phase && (m->steps_in_call == 0 || m->tile_steps % kPhaseEvery == 0);
   FS_TRY(kt_begin(m, FS_K_TILE_ACC));
   if (halo == kHalo) {
-    hipLaunchKernelGGL(k_tile_acc, dim3(tiles), dim3(kAccBlock), 0, s, m->rcv, m->cnt, m->lvl,
-                       m->plan, m->hdr, m->W, m->H, tiles_x, tiles);
+    if (timed)
+      hipLaunchKernelGGL(k_tile_acc<true>, dim3(tiles), dim3(kAccBlock), 0, s, m->rcv, m->cnt,
+                         m->lvl, m->plan, m->hdr, m->W, m->H, tiles_x, tiles, m->phase);
+    else
+      hipLaunchKernelGGL(k_tile_acc<false>, dim3(tiles), dim3(kAccBlock), 0, s, m->rcv, m->cnt,
+                         m->lvl, m->plan, m->hdr, m->W, m->H, tiles_x, tiles, m->phase);
   } else {
     if (!m->plan32) {
       if (!m->owns_memory) return FS_ERR_INTERNAL;   // lent buffers include plan32
       FS_TRY(fs::dev_alloc(&m->plan32, static_cast<size_t>(tiles)));
     }
-    hipLaunchKernelGGL(tile32::k_tile_acc, dim3(tiles), dim3(tile32::kAccBlock), 0, s, m->rcv,
-                       m->cnt, m->lvl, m->plan32, m->hdr, m->W, m->H, tiles_x, tiles);
+    if (timed)
+      hipLaunchKernelGGL(tile32::k_tile_acc<true>, dim3(tiles), dim3(tile32::kAccBlock), 0, s,
+                         m->rcv, m->cnt, m->lvl, m->plan32, m->hdr, m->W, m->H, tiles_x, tiles,
+                         m->phase);
+    else
+      hipLaunchKernelGGL(tile32::k_tile_acc<false>, dim3(tiles), dim3(tile32::kAccBlock), 0, s,
+                         m->rcv, m->cnt, m->lvl, m->plan32, m->hdr, m->W, m->H, tiles_x, tiles,
+                         m->phase);
   }
   FS_TRY(check_launch());
   FS_TRY(kt_end(m));
@@ -610,9 +702,9 @@ int tile_attempt(fs_model *m, int halo, int tiles, int tiles_x, bool clear, hipE
   FS_TRY(mark(ev[4], s));
   FS_TRY(mark(ev[5], s));
   FS_TRY(kt_begin(m, FS_K_TILE_ERODE));
-  if (m->nmode == 1) FS_TRY(launch_tile_erode<1>(m, tiles, tiles_x, halo));
-  else if (m->nmode == 2) FS_TRY(launch_tile_erode<2>(m, tiles, tiles_x, halo));
-  else FS_TRY(launch_tile_erode<0>(m, tiles, tiles_x, halo));
+  if (m->nmode == 1) FS_TRY(launch_tile_erode<1>(m, tiles, tiles_x, halo, timed));
+  else if (m->nmode == 2) FS_TRY(launch_tile_erode<2>(m, tiles, tiles_x, halo, timed));
+  else FS_TRY(launch_tile_erode<0>(m, tiles, tiles_x, halo, timed));
   FS_TRY(kt_end(m));
   if (m->side) {
     FS_HIP_TRY(hipStreamSynchronize(m->side));
@@ -652,6 +744,7 @@ int tile_path(fs_model *m, hipEvent_t *ev, bool *done) {
   // a 32-halo step whose levels all fit 16 tries the 16-cell halo next
   m->tile_halo = (halo == tile32::kHalo && m->hdr_h[0] <= kHalo) ? kHalo : halo;
   m->order_current = false;                 // levels, stack, donor masks on request
+  m->cnt_stack = false;
   std::swap(m->h, m->h_alt);
   m->tile_steps++;
   if (halo != kHalo) m->tile32_steps++;
@@ -659,9 +752,43 @@ int tile_path(fs_model *m, hipEvent_t *ev, bool *done) {
   return FS_OK;
 }
 
+// Depression filling of the current field (an extension; the reference has
+// none): the Priority-Flood+epsilon surface, relaxed tile by tile into the
+// second buffer until a launch changes no cell, then made current.  Each
+// launch finalises every cell whose flood path crosses one more tile
+// border, so the launches are bounded.
+int fill_depressions(fs_model *m) {
+  if (!m->fill_flag) return FS_ERR_INVALID_ARGUMENT;   // models over lent buffers
+  hipStream_t s = m->stream;
+  const size_t N = static_cast<size_t>(m->N);
+  const unsigned ib = static_cast<unsigned>(std::min<size_t>((N + 255) / 256, 1u << 16));
+  hipLaunchKernelGGL(k_fill_init, dim3(ib), dim3(256), 0, s, m->h, m->h_alt, m->W, m->H);
+  FS_TRY(check_launch());
+  const int tiles_x = (m->W + kFillTile - 1) / kFillTile;
+  const int tiles = tiles_x * ((m->H + kFillTile - 1) / kFillTile);
+  const long long cap = 8ll * tiles + 16;
+  for (long long launch = 0;; launch++) {
+    if (launch > cap) {
+      fs::set_last_error("depression filling did not settle");
+      return FS_ERR_INTERNAL;
+    }
+    FS_HIP_TRY(hipMemsetAsync(m->fill_flag, 0, sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_fill, dim3(tiles), dim3(kFillBlock), 0, s, m->h, m->h_alt, m->W, m->H,
+                       tiles_x, m->fill_flag);
+    FS_TRY(check_launch());
+    int32_t changed = 0;
+    FS_HIP_TRY(hipMemcpyAsync(&changed, m->fill_flag, sizeof(changed), hipMemcpyDeviceToHost, s));
+    FS_HIP_TRY(hipStreamSynchronize(s));
+    if (!changed) break;
+  }
+  std::swap(m->h, m->h_alt);
+  return FS_OK;
+}
+
 // One time-step.
 int step_once(fs_model *m) {
   hipEvent_t *ev = m->ev[m->ev_cur];
+  if (m->fill_mode) FS_TRY(fill_depressions(m));
   FS_TRY(mark(ev[0], m->stream));
   FS_TRY(launch_receivers(m));
   FS_TRY(mark(ev[1], m->stream));
@@ -675,6 +802,7 @@ int step_once(fs_model *m) {
   }
   FS_TRY(mark(ev[6], m->stream));
   m->ev_pending[m->ev_cur] = ev[0] != nullptr;
+  m->ev_tile[m->ev_cur] = done;
   m->ev_cur ^= 1;
   m->have_step = true;
   return FS_OK;
@@ -703,9 +831,14 @@ int run_steps(fs_model *m, long long iterations) {
     FS_HIP_TRY(hipMemcpy(m->hdr + 2, &zero, sizeof(zero), hipMemcpyHostToDevice));
   }
   int st = FS_OK;
-  for (long long i = 0; i < iterations && st == FS_OK; i++) st = step_once(m);
+  for (long long i = 0; i < iterations && st == FS_OK; i++) {
+    m->steps_in_call = i;
+    st = step_once(m);
+  }
   const int fst = finish(m);
   if (st == FS_OK) st = fst;
+  const int sst = split_tile_time(m);
+  if (st == FS_OK) st = sst;
   for (int i = 0; i < 6; i++) {
     const double ms = m->stage_ms[FS_T_RECEIVERS + i];
     const uint64_t whole = static_cast<uint64_t>(ms * 1000.0);
@@ -876,7 +1009,19 @@ int fs_get_accum(fs_model *m, double *accum, size_t count) {
   if (!accum) return FS_ERR_INVALID_ARGUMENT;
   if (count != static_cast<size_t>(m->N)) return FS_ERR_BUFFER_SIZE;
   std::vector<uint32_t> c(count);
-  FS_TRY(download(m, c.data(), m->cnt, count));
+  if (m->cnt_stack) {
+    // the level path keeps counts in stack order: scatter them by cell
+    std::vector<int32_t> st(count);
+    std::vector<uint32_t> cs(count);
+    FS_TRY(download(m, st.data(), m->stack, count));
+    FS_TRY(download(m, cs.data(), m->cnt_s, count));
+    for (size_t p = 0; p < count; p++) {
+      if (static_cast<uint32_t>(st[p]) >= count) return FS_ERR_INTERNAL;
+      c[static_cast<size_t>(st[p])] = cs[p];
+    }
+  } else {
+    FS_TRY(download(m, c.data(), m->cnt, count));
+  }
   for (size_t i = 0; i < count; i++) accum[i] = static_cast<double>(c[i]) * m->p.cell_area;
   return FS_OK;
 }
@@ -951,6 +1096,21 @@ int fs_calibrate_copy(size_t nbytes, int device, double *ms) {
   return st;
 }
 
+int fs_fill_depressions(fs_model *m) {
+  FS_TRY(check_model(m));
+  if (!m->owns_memory) return FS_ERR_INVALID_ARGUMENT;
+  FS_HIP_TRY(hipStreamSynchronize(m->stream));
+  FS_TRY(fill_depressions(m));
+  FS_HIP_TRY(hipStreamSynchronize(m->stream));
+  return FS_OK;
+}
+
+int fs_set_depression_fill(fs_model *m, int enable) {
+  if (!m || !m->owns_memory) return FS_ERR_INVALID_ARGUMENT;
+  m->fill_mode = enable != 0;
+  return FS_OK;
+}
+
 int fs_set_step_path(fs_model *m, int path) {
   if (!m || (path != FS_PATH_AUTO && path != FS_PATH_LEVELS)) return FS_ERR_INVALID_ARGUMENT;
   m->path = path;
@@ -1001,8 +1161,8 @@ constexpr size_t na_align(size_t v) { return (v + 255) & ~size_t{255}; }
 
 // Byte offsets of the work arrays in `scratch` (each 256-byte aligned).
 struct NaLayout {
-  size_t h_alt, rcv, dmask, lvl, depth, jump, list, stack, cnt, ftab, hdr, levels, status,
-      ticket, plan, plan32, total;
+  size_t h_alt, rcv, dmask, lvl, depth, jump, list, stack, cnt, rdir, fdon, ndon, cnt_s, ftab,
+      hdr, levels, status, ticket, plan, plan32, total;
 };
 
 constexpr NaLayout na_layout() {
@@ -1018,6 +1178,10 @@ constexpr NaLayout na_layout() {
   L.list = o;   o = na_align(o + 4 * n);
   L.stack = o;  o = na_align(o + 4 * n);
   L.cnt = o;    o = na_align(o + 4 * n);
+  L.rdir = o;   o = na_align(o + n);
+  L.fdon = o;   o = na_align(o + 4 * n);
+  L.ndon = o;   o = na_align(o + n);
+  L.cnt_s = o;  o = na_align(o + 4 * n);
   L.ftab = o;   o = na_align(o + 16 * ((n > kFactCache - 1 ? n : kFactCache - 1) + 1));
   L.hdr = o;    o = na_align(o + 4 * static_cast<size_t>(kHdrAlloc));
   L.levels = o; o = na_align(o + 4 * static_cast<size_t>(FS_MAX_LEVELS + 4));
@@ -1029,7 +1193,7 @@ constexpr NaLayout na_layout() {
   return L;
 }
 constexpr NaLayout kNa = na_layout();
-static_assert(kNa.total <= (size_t{64} << 20), "scratch stays within 64 MiB");
+static_assert(kNa.total <= (size_t{76} << 20), "scratch stays within 76 MiB");
 
 __device__ __align__(256) unsigned char scratch[kNa.total];
 
@@ -1088,6 +1252,10 @@ int na_setup(int dev, NaDevice &d) {
   m->list = reinterpret_cast<int32_t *>(b + kNa.list);
   m->stack = reinterpret_cast<int32_t *>(b + kNa.stack);
   m->cnt = reinterpret_cast<uint32_t *>(b + kNa.cnt);
+  m->rdir = b + kNa.rdir;
+  m->fdon = reinterpret_cast<int32_t *>(b + kNa.fdon);
+  m->ndon = b + kNa.ndon;
+  m->cnt_s = reinterpret_cast<uint32_t *>(b + kNa.cnt_s);
   m->ftab = reinterpret_cast<double *>(b + kNa.ftab);
   m->hdr = reinterpret_cast<int32_t *>(b + kNa.hdr);
   m->levels_d = reinterpret_cast<int32_t *>(b + kNa.levels);
@@ -1140,7 +1308,8 @@ extern "C" int fs_step_noalloc(double *h, int width, int height, const fs_params
   m->stream = s;
   m->h = h;
   m->h_alt = reinterpret_cast<double *>(d.base + kNa.h_alt);
-  m->have_step = m->order_current = false;
+  m->have_step = m->order_current = m->cnt_stack = false;
+  m->fill_mode = false;
   m->path = tiles <= kNaMaxTiles ? FS_PATH_AUTO : FS_PATH_LEVELS;
   m->tile_halo = kHalo;
   m->prev_maxdepth = FS_MAX_LEVELS;

@@ -187,14 +187,15 @@ __global__ void k_fill_counts(int32_t *hdr, int n_counts) {
 // elevation is fetched from HBM about once (neighbouring columns come from
 // L1).  fs::receiver_fast picks the reference's direction -- steepest
 // (hc - h[n]) / dr[n] over n = 0..7 with a strict '>' from 0, ties to the
-// lowest n, flat cells and pits NO_FLOW -- with two IEEE divisions instead of
-// eight (see fs_receiver.h).
+// lowest n, flat cells and pits NO_FLOW -- usually without dividing, else
+// with two IEEE divisions instead of eight (see fs_receiver.h).
 constexpr int kRecvRows = 16;
 __device__ __forceinline__ void receivers_strip(const double *__restrict__ h,
                                                 uint8_t *__restrict__ rcv, int W, int H, int x,
                                                 int strip, double d_card, double d_diag) {
   const int y0 = strip * kRecvRows + 2;
   const int y1 = min(y0 + kRecvRows, H - 2);
+  const bool div_ok = fs::divisors_in_range(d_card, d_diag);
   const double *p = h + static_cast<size_t>(y0 - 1) * W + x;
   double a0 = p[-1], a1 = p[0], a2 = p[1];        // row y-1
   p += W;
@@ -205,7 +206,7 @@ __device__ __forceinline__ void receivers_strip(const double *__restrict__ h,
     // neighbours in direction order W, NW, N, NE, E, SE, S, SW (RB:79-82)
     const double nb[8] = {b0, a0, a1, a2, b2, c2, c1, c0};
     rcv[static_cast<size_t>(y) * W + x] =
-        static_cast<uint8_t>(fs::receiver_fast(b1, nb, d_card, d_diag));
+        static_cast<uint8_t>(fs::receiver_fast(b1, nb, d_card, d_diag, div_ok));
     a0 = b0; a1 = b1; a2 = b2;
     b0 = c0; b1 = c1; b2 = c2;
   }
@@ -404,12 +405,43 @@ __global__ __launch_bounds__(kScanBlock) void k_roots(
   }
 }
 
+// Stack-order records of the level path, written as the breadth-first
+// order is built: for stack position p, fdon[p] is where its donor list was
+// appended and ndon[p] its length; for every appended donor, rdir is its
+// receiver direction 0..7 (RB:79-82).  The donor in mask slot j drains in
+// direction kSlotDir[j] (slot 0, the north-west neighbour, drains south-east
+// and so on).
+__device__ __forceinline__ int slot_dir(int j) {
+  constexpr uint32_t kSlotDir = 5u | 6u << 3 | 7u << 6 | 4u << 9 | 0u << 12 | 3u << 15 |
+                                2u << 18 | 1u << 21;
+  return static_cast<int>((kSlotDir >> (3 * j)) & 7u);
+}
+
+__device__ __forceinline__ void append_donors(int32_t c, uint32_t m, int32_t p, long long pos,
+                                              int32_t *__restrict__ stack,
+                                              uint8_t *__restrict__ rdir,
+                                              int32_t *__restrict__ fdon,
+                                              uint8_t *__restrict__ ndon, int W, int32_t N) {
+  fdon[p] = static_cast<int32_t>(pos < N ? pos : N);
+  ndon[p] = static_cast<uint8_t>(__popc(m));
+  while (m) {
+    const int j = __builtin_ctz(m);
+    m &= m - 1;
+    if (pos < N) {
+      stack[pos] = c + slot_offset(j, W);
+      rdir[pos] = static_cast<uint8_t>(slot_dir(j));
+    }
+    pos++;
+  }
+}
+
 // One BFS level of GenerateOrder (RB:234-247): stack[ls, le) -> the donor
 // lists of those cells, in order, written from stack[le].
 __global__ __launch_bounds__(kScanBlock) void k_bfs(
-    int32_t *__restrict__ stack, const uint8_t *__restrict__ dmask, int32_t ls, int32_t le,
-    int W, int32_t N, uint64_t *status, unsigned long long *ticket,
-    unsigned long long tbase, uint32_t epoch, int32_t *hdr) {
+    int32_t *__restrict__ stack, const uint8_t *__restrict__ dmask, uint8_t *__restrict__ rdir,
+    int32_t *__restrict__ fdon, uint8_t *__restrict__ ndon, int32_t ls, int32_t le, int W,
+    int32_t N, uint64_t *status, unsigned long long *ticket, unsigned long long tbase,
+    uint32_t epoch, int32_t *hdr) {
   __shared__ uint32_t sh[kScanBlock / kWave + 1];
   __shared__ uint32_t s_tile, s_pre;
   if (threadIdx.x == 0) s_tile = static_cast<uint32_t>(atomicAdd(ticket, 1ull) - tbase);
@@ -444,20 +476,19 @@ __global__ __launch_bounds__(kScanBlock) void k_bfs(
   long long pos = static_cast<long long>(le) + s_pre + ex;
 #pragma unroll
   for (int i = 0; i < kBfsItems; i++) {
-    uint32_t m = static_cast<uint32_t>(masks >> (8 * i)) & 0xFFu;
-    while (m) {
-      const int j = __builtin_ctz(m);
-      m &= m - 1;
-      if (pos < N) stack[pos] = cell[i] + slot_offset(j, W);
-      pos++;
-    }
+    const long long p = p0 + i;
+    if (p >= le) break;
+    const uint32_t m = static_cast<uint32_t>(masks >> (8 * i)) & 0xFFu;
+    append_donors(cell[i], m, static_cast<int32_t>(p), pos, stack, rdir, fdon, ndon, W, N);
+    pos += __popc(m);
   }
 }
 
 // Levels [L0, L1) of the BFS in one workgroup, for runs of small levels.
 __global__ __launch_bounds__(kSmallBlock) void k_bfs_small(
-    int32_t *__restrict__ stack, const uint8_t *__restrict__ dmask,
-    const int32_t *__restrict__ levels, int L0, int L1, int W, int32_t N, int32_t *hdr) {
+    int32_t *__restrict__ stack, const uint8_t *__restrict__ dmask, uint8_t *__restrict__ rdir,
+    int32_t *__restrict__ fdon, uint8_t *__restrict__ ndon, const int32_t *__restrict__ levels,
+    int L0, int L1, int W, int32_t N, int32_t *hdr) {
   __shared__ uint32_t sh[kSmallBlock / kWave + 1];
   for (int L = L0; L < L1; L++) {
     const int32_t ls = levels[L], le = levels[L + 1];
@@ -485,13 +516,11 @@ __global__ __launch_bounds__(kSmallBlock) void k_bfs_small(
       long long pos = out + ex;
 #pragma unroll
       for (int i = 0; i < kSmallItems; i++) {
-        uint32_t m = (masks >> (8 * i)) & 0xFFu;
-        while (m) {
-          const int j = __builtin_ctz(m);
-          m &= m - 1;
-          if (pos < N) stack[pos] = cell[i] + slot_offset(j, W);
-          pos++;
-        }
+        const long long p = p0 + i;
+        if (p >= le) break;
+        const uint32_t m = (masks >> (8 * i)) & 0xFFu;
+        append_donors(cell[i], m, static_cast<int32_t>(p), pos, stack, rdir, fdon, ndon, W, N);
+        pos += __popc(m);
       }
       out += total;
     }
@@ -499,42 +528,50 @@ __global__ __launch_bounds__(kSmallBlock) void k_bfs_small(
   }
 }
 
-// Upstream cell count of one cell: 1 + the counts of its donors, which sit
-// one level higher and are already final.  accum = count * cell_area exactly
-// (RB:262-279: every partial sum there is an integer multiple of cell_area).
-__device__ __forceinline__ void flowacc_cell(int32_t c, const uint8_t *__restrict__ dmask,
-                                             uint32_t *cnt, int W, int32_t N, int32_t *hdr) {
-  if (static_cast<uint32_t>(c) >= static_cast<uint32_t>(N)) {
-    atomicOr(&hdr[2], kErrIndex);
+// Upstream cell count of stack entry p: 1 + the counts of its donors, one
+// level higher, already final and contiguous from fdon[p].  Counts are kept
+// in stack order (cnt_s); accum = count * cell_area exactly (RB:262-279:
+// every partial sum there is an integer multiple of cell_area).  Entries of
+// the deepest level have no donors (the breadth-first pass never records
+// them): top is true for those.
+__device__ __forceinline__ void flowacc_entry(int32_t p, bool top,
+                                              const int32_t *__restrict__ fdon,
+                                              const uint8_t *__restrict__ ndon, uint32_t *cnt_s,
+                                              int32_t N, int32_t *hdr) {
+  if (top) {
+    cnt_s[p] = 1;
     return;
   }
-  uint32_t m = dmask[c];
+  const int32_t f = fdon[p];
+  const int k = ndon[p];
   uint32_t s = 1;
-  while (m) {
-    const int j = __builtin_ctz(m);
-    m &= m - 1;
-    s += cnt[c + slot_offset(j, W)];
+  if (f > p && static_cast<long long>(f) + k <= N) {
+    for (int i = 0; i < k; i++) s += cnt_s[f + i];
+  } else if (k > 0) {
+    atomicOr(&hdr[2], kErrIndex);
   }
-  cnt[c] = s;
+  cnt_s[p] = s;
 }
 
-__global__ __launch_bounds__(256) void k_flowacc(const int32_t *__restrict__ stack,
-                                                 const uint8_t *__restrict__ dmask,
-                                                 uint32_t *cnt, int32_t ls, int32_t le, int W,
-                                                 int32_t N, int32_t *hdr) {
+__global__ __launch_bounds__(256) void k_flowacc(const int32_t *__restrict__ fdon,
+                                                 const uint8_t *__restrict__ ndon,
+                                                 uint32_t *cnt_s, int32_t ls, int32_t le,
+                                                 bool top, int32_t N, int32_t *hdr) {
   const long long p = static_cast<long long>(ls) + blockIdx.x * 256ll + threadIdx.x;
   if (p >= le) return;
-  flowacc_cell(stack[p], dmask, cnt, W, N, hdr);
+  flowacc_entry(static_cast<int32_t>(p), top, fdon, ndon, cnt_s, N, hdr);
 }
 
-// Levels L_hi down to L_lo (inclusive) in one workgroup.
+// Levels L_hi down to L_lo (inclusive) in one workgroup; L_top is the
+// deepest level of the step.
 __global__ __launch_bounds__(kSmallBlock) void k_flowacc_small(
-    const int32_t *__restrict__ stack, const uint8_t *__restrict__ dmask, uint32_t *cnt,
-    const int32_t *__restrict__ levels, int L_hi, int L_lo, int W, int32_t N, int32_t *hdr) {
+    const int32_t *__restrict__ fdon, const uint8_t *__restrict__ ndon, uint32_t *cnt_s,
+    const int32_t *__restrict__ levels, int L_hi, int L_lo, int L_top, int32_t N,
+    int32_t *hdr) {
   for (int L = L_hi; L >= L_lo; L--) {
     const int32_t ls = levels[L], le = levels[L + 1];
     for (long long p = ls + threadIdx.x; p < le; p += kSmallBlock)
-      flowacc_cell(stack[p], dmask, cnt, W, N, hdr);
+      flowacc_entry(static_cast<int32_t>(p), L == L_top, fdon, ndon, cnt_s, N, hdr);
     __syncthreads();
   }
 }
@@ -619,53 +656,189 @@ __device__ __forceinline__ double newton_update(double h0, double hn, double fac
   return hnew;
 }
 
-// Erode for one stack entry (RB:306-328): h[c] from h[c] + U*dt and the
-// receiver's final elevation (its level was processed earlier).
+// Erode for stack entry p (RB:306-328): the cell's new elevation from
+// h + U*dt and its receiver's final elevation (one level lower, so already
+// written; it sits next to the cell in the grid, usually in the same cache
+// line).  The count and the receiver direction come from stack order.
 template <int NMODE>
-__device__ __forceinline__ void erode_cell(int32_t c, const uint8_t *__restrict__ rcv,
-                                           const uint32_t *__restrict__ cnt,
-                                           const double *__restrict__ ftab, double *h,
-                                           int W, int32_t N, const ErodeConst &k,
-                                           int32_t *hdr) {
-  if (static_cast<uint32_t>(c) >= static_cast<uint32_t>(N)) {
+__device__ __forceinline__ void erode_entry(int32_t p, const int32_t *__restrict__ stack,
+                                            const uint8_t *__restrict__ rdir,
+                                            const uint32_t *__restrict__ cnt_s,
+                                            const double *__restrict__ ftab, double *h, int W,
+                                            int32_t N, const ErodeConst &k, int32_t *hdr) {
+  const int32_t c = stack[p];
+  const int r = rdir[p] & 7;
+  uint32_t a = cnt_s[p];
+  const int32_t nb = c + dir_offset(r, W);
+  if (static_cast<uint32_t>(c) >= static_cast<uint32_t>(N) ||
+      static_cast<uint32_t>(nb) >= static_cast<uint32_t>(N) || a > static_cast<uint32_t>(N)) {
     atomicOr(&hdr[2], kErrIndex);
     return;
   }
-  const int r = rcv[c];
-  if (r >= 8) return;
-  const int32_t n = c + dir_offset(r, W);
-  uint32_t a = cnt[c];
-  if (a > static_cast<uint32_t>(N)) {
-    atomicOr(&hdr[2], kErrIndex);
-    a = static_cast<uint32_t>(N);
-  }
-  h[c] = newton_update<NMODE>(h[c] + k.ud, h[n], ftab[2 * a + (r & 1)], k, hdr);
+  h[c] = newton_update<NMODE>(h[c] + k.ud, h[nb], ftab[2 * a + (r & 1)], k, hdr);
 }
 
 template <int NMODE>
 __global__ __launch_bounds__(256) void k_erode(const int32_t *__restrict__ stack,
-                                               const uint8_t *__restrict__ rcv,
-                                               const uint32_t *__restrict__ cnt,
+                                               const uint8_t *__restrict__ rdir,
+                                               const uint32_t *__restrict__ cnt_s,
                                                const double *__restrict__ ftab, double *h,
                                                int32_t ls, int32_t le, int W, int32_t N,
                                                ErodeConst k, int32_t *hdr) {
   const long long p = static_cast<long long>(ls) + blockIdx.x * 256ll + threadIdx.x;
   if (p >= le) return;
-  erode_cell<NMODE>(stack[p], rcv, cnt, ftab, h, W, N, k, hdr);
+  erode_entry<NMODE>(static_cast<int32_t>(p), stack, rdir, cnt_s, ftab, h, W, N, k, hdr);
 }
 
 template <int NMODE>
 __global__ __launch_bounds__(kSmallBlock) void k_erode_small(
-    const int32_t *__restrict__ stack, const uint8_t *__restrict__ rcv,
-    const uint32_t *__restrict__ cnt, const double *__restrict__ ftab, double *h,
+    const int32_t *__restrict__ stack, const uint8_t *__restrict__ rdir,
+    const uint32_t *__restrict__ cnt_s, const double *__restrict__ ftab, double *h,
     const int32_t *__restrict__ levels, int L_lo, int L_hi, int W, int32_t N, ErodeConst k,
     int32_t *hdr) {
   for (int L = L_lo; L <= L_hi; L++) {
     const int32_t ls = levels[L], le = levels[L + 1];
     for (long long p = ls + threadIdx.x; p < le; p += kSmallBlock)
-      erode_cell<NMODE>(stack[p], rcv, cnt, ftab, h, W, N, k, hdr);
+      erode_entry<NMODE>(static_cast<int32_t>(p), stack, rdir, cnt_s, ftab, h, W, N, k, hdr);
     __syncthreads();
   }
+}
+
+// ---------------------------------------------------------------------------
+// depression filling (an extension: the reference has none)
+// ---------------------------------------------------------------------------
+// The Priority-Flood+epsilon surface (Barnes, Lehman & Mulla 2014): the
+// unique fixed point of F = max(h, nextafter(min over the D8 neighbours of F,
+// +inf)) on interior cells, rings 0 and 1 held at h.  Starting from +inf the
+// update only lowers F, so tiles relax it in LDS against their neighbours'
+// last values (reads may see older, higher values: the iteration still
+// decreases to the fixed point); the host relaunches until a launch changes
+// nothing, which leaves every cell consistent with its neighbours' final
+// values.
+constexpr int kFillTile = 64;
+constexpr int kFillReg = kFillTile + 2;
+constexpr int kFillBlock = 256;
+constexpr int kFillSweeps = 512;          // local sweeps per launch at most
+
+// nextafter(x, +inf) for IEEE doubles.
+__device__ __forceinline__ double next_up(double x) {
+  if (!(x < __builtin_huge_val())) return x;          // +inf, NaN
+  if (x == 0.0) return 0x1p-1074;                      // +-0 -> smallest subnormal
+  const long long u = __double_as_longlong(x);
+  return __longlong_as_double(x > 0.0 ? u + 1 : u - 1);
+}
+
+// F = h on rings 0 and 1, +inf inside.
+__global__ __launch_bounds__(256) void k_fill_init(const double *__restrict__ h,
+                                                   double *__restrict__ F, int W, int H) {
+  const long long N = static_cast<long long>(W) * H;
+  for (long long c = blockIdx.x * 256ll + threadIdx.x; c < N; c += gridDim.x * 256ll) {
+    const int y = static_cast<int>(c / W), x = static_cast<int>(c - static_cast<long long>(y) * W);
+    const bool inner = x >= 2 && x < W - 2 && y >= 2 && y < H - 2;
+    F[c] = inner ? __builtin_huge_val() : h[c];
+  }
+}
+
+__global__ __launch_bounds__(kFillBlock) void k_fill(const double *__restrict__ h, double *F,
+                                                     int W, int H, int tiles_x,
+                                                     int32_t *changed) {
+  __shared__ double f_s[kFillReg * kFillReg];
+  __shared__ int s_more, s_any;
+  constexpr int kPer = kFillTile * kFillTile / kFillBlock;
+  const int x0 = (blockIdx.x % tiles_x) * kFillTile, y0 = (blockIdx.x / tiles_x) * kFillTile;
+  for (int i = threadIdx.x; i < kFillReg * kFillReg; i += kFillBlock) {
+    const int gx = x0 - 1 + i % kFillReg, gy = y0 - 1 + i / kFillReg;
+    f_s[i] = (gx >= 0 && gx < W && gy >= 0 && gy < H)
+                 ? __hip_atomic_load(&F[static_cast<size_t>(gy) * W + gx], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT)
+                 : __builtin_huge_val();
+  }
+  double hv[kPer];
+  bool inner[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const int i = k * kFillBlock + threadIdx.x;
+    const int gx = x0 + i % kFillTile, gy = y0 + i / kFillTile;
+    inner[k] = gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2;
+    hv[k] = inner[k] ? h[static_cast<size_t>(gy) * W + gx] : 0.0;
+  }
+  if (threadIdx.x == 0) s_any = 0;
+  __syncthreads();
+  for (int sweep = 0; sweep < kFillSweeps; sweep++) {
+    if (threadIdx.x == 0) s_more = 0;
+    __syncthreads();
+    int more = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      if (!inner[k]) continue;
+      const int i = k * kFillBlock + threadIdx.x;
+      const int r = (i / kFillTile + 1) * kFillReg + i % kFillTile + 1;
+      double m = f_s[r - kFillReg - 1];
+      m = fmin(m, f_s[r - kFillReg]);
+      m = fmin(m, f_s[r - kFillReg + 1]);
+      m = fmin(m, f_s[r - 1]);
+      m = fmin(m, f_s[r + 1]);
+      m = fmin(m, f_s[r + kFillReg - 1]);
+      m = fmin(m, f_s[r + kFillReg]);
+      m = fmin(m, f_s[r + kFillReg + 1]);
+      const double v = fmax(hv[k], next_up(m));
+      if (v < f_s[r]) {
+        f_s[r] = v;
+        more = 1;
+      }
+    }
+    if (more) s_more = 1;
+    __syncthreads();
+    const int go = s_more;
+    if (go && threadIdx.x == 0) s_any = 1;
+    __syncthreads();
+    if (!go) break;
+  }
+  if (!s_any) return;
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    if (!inner[k]) continue;
+    const int i = k * kFillBlock + threadIdx.x;
+    const int gx = x0 + i % kFillTile, gy = y0 + i / kFillTile;
+    const int r = (i / kFillTile + 1) * kFillReg + i % kFillTile + 1;
+    __hip_atomic_store(&F[static_cast<size_t>(gy) * W + gx], f_s[r], __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (threadIdx.x == 0) atomicOr(changed, 1);
+}
+
+// ---------------------------------------------------------------------------
+// phase timing of the tile kernels
+// ---------------------------------------------------------------------------
+// The tile path fuses the reference's stages (donors, order, accumulation,
+// uplift, erosion; RB:367-372) into two kernels.  To report the reference's
+// per-stage timers, the timed instantiations of the tile kernels (run on a
+// sample of steps) let one workgroup in kPhaseSample add the wall-clock
+// ticks of each of its phases (between barriers, never inside a loop:
+// reading the clock is a slow scalar-memory access) to a row of `phase`
+// (rows spread the atomics); the host splits the kernels' event times over
+// the stages in proportion.
+constexpr int kPhaseSample = 16;     // workgroups per sampled one in a timed launch
+constexpr int kPhaseEvery = 32;      // tile steps per timed step
+constexpr int kPhaseSlots = 32;
+constexpr int kPhaseStride = 8;
+enum PhaseId {
+  kPhAccLoad = 0,    // receivers -> region words (the donor structure)   Donors
+  kPhAccWalk,        // receiver-chain walks: counts and levels          FlowAcc
+  kPhAccPlan,        // halo chains, slot scan, erosion plan             Order
+  kPhAccStore,       // count and level stores                           FlowAcc
+  kPhEroBucket,      // loads and level bucketing of the slots           Order
+  kPhEroRoots,       // uplift of the outlets                            Uplift
+  kPhEroLoop,        // the erosion wavefront                            Erosion
+  kPhEroStore,       // elevation stores                                 Erosion
+  kPhCount
+};
+static_assert(kPhCount <= kPhaseStride, "phase row");
+
+// This workgroup's phase row, or null when it is not sampled.
+__device__ __forceinline__ unsigned long long *phase_row(unsigned long long *phase) {
+  if (!phase || blockIdx.x % kPhaseSample != 0) return nullptr;
+  return phase + (blockIdx.x / kPhaseSample % kPhaseSlots) * kPhaseStride;
 }
 
 // ---------------------------------------------------------------------------

@@ -18,10 +18,11 @@
 //   E  work items claimed in list order from one counter:
 //        shallow: tile uplift + erosion in LDS (RB:285-328)
 //        deep:    a level path run by the realization's workgroup group:
-//                 donor masks and outlets, BFS levels (RB:213-254), counts
-//                 pulled from the top level down (RB:262-279), then uplift
-//                 and erosion level by level (RB:285-328), with group
-//                 barriers between levels
+//                 donor masks and outlets, BFS levels (RB:213-254) that
+//                 record each stack entry's receiver and donor positions,
+//                 counts pulled from the top level down (RB:262-279) and
+//                 uplift and erosion level by level (RB:285-328) in stack
+//                 order, with group barriers between levels
 //
 // Results do not depend on the schedule (work item sizes, group sizes): each
 // cell's elevation depends only on its own inputs and its receiver's final
@@ -96,6 +97,11 @@ struct EnsArgs {
   uint32_t *cnt;
   TilePlan *plan;
   int32_t *order, *lev;
+  // the level path's records in stack order (index = position in `order`)
+  uint8_t *rdir;      // receiver direction
+  int32_t *fdon;      // position of the first donor
+  uint8_t *ndon;      // number of donors
+  uint32_t *cnt_s;    // upstream cell count
   const double *ftab;
   int32_t *rh, *hist, *ctl;
   const int32_t *recv_first;   // R+1 prefix of receiver work items
@@ -211,7 +217,11 @@ __device__ void append_entries(int R, Pick pick, Count count, int32_t *off, int3
 // outlets; a level's entries are appended in any order (the stack order of
 // GenerateOrder is not needed: counts are integers and each elevation
 // depends only on its receiver's), so every level's size -- the levels[]
-// offsets -- matches the reference's.
+// offsets -- matches the reference's.  The breadth-first pass records, for
+// every stack position, its receiver direction and where its donors were
+// appended, so counts are pulled from contiguous runs and the erosion reads
+// count and direction contiguously; only the cell's elevation and its
+// receiver's (usually in the same cache line) are read from the grid.
 template <int NMODE>
 __device__ void deep_path(const EnsArgs &a, const EnsReal &g, int r, int member, int G,
                           DeepShared &sd) {
@@ -223,7 +233,10 @@ __device__ void deep_path(const EnsArgs &a, const EnsReal &g, int r, int member,
   const uint8_t *rc = a.rcv + g.off;
   uint8_t *dm = a.dmask + g.off;
   int32_t *ord = a.order + g.off;
-  uint32_t *ct = a.cnt + g.off;
+  uint8_t *rdir = a.rdir + g.off;
+  int32_t *fdon = a.fdon + g.off;
+  uint8_t *ndon = a.ndon + g.off;
+  uint32_t *cs = a.cnt_s + g.off;
   const double *ho = a.h_old + g.off;
   double *hn = a.h_new + g.off;
   int32_t *lev = a.lev + g.lev_off;
@@ -266,7 +279,8 @@ __device__ void deep_path(const EnsArgs &a, const EnsReal &g, int r, int member,
   }
   if (t == 0 && G > 1) __hip_atomic_store(ctr + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   group_barrier(G, q + kRhBar, bar, a.ctl);
-  // breadth-first levels: level L is ord[lo, hi); its donors become level L+1
+  // breadth-first levels: level L is ord[lo, hi); its donors become level
+  // L+1, each donor list contiguous from fdon[p]
   int lo = 0, hi = ld_agent(ctr + 0);
   if (t == 0) lev[0] = 0;
   int L = 0;
@@ -289,12 +303,21 @@ __device__ void deep_path(const EnsArgs &a, const EnsReal &g, int r, int member,
         if (static_cast<uint32_t>(c) < static_cast<uint32_t>(n)) m = dm[c];
         else atomicOr(&q[kRhErr], kErrIndex);
       }
-      int pos = hi + wave_reserve(cl, __popc(m));
+      const int k = __popc(m);
+      int pos = hi + wave_reserve(cl, k);
+      if (p < hi) {
+        fdon[p] = pos;
+        ndon[p] = static_cast<uint8_t>(k);
+      }
       while (m) {
         const int j = __builtin_ctz(m);
         m &= m - 1;
-        if (pos < n) ord[pos] = c + slot_offset(j, W);
-        else atomicOr(&q[kRhErr], kErrIndex);
+        if (pos < n) {
+          ord[pos] = c + slot_offset(j, W);
+          rdir[pos] = static_cast<uint8_t>(slot_dir(j));
+        } else {
+          atomicOr(&q[kRhErr], kErrIndex);
+        }
         pos++;
       }
     }
@@ -310,22 +333,24 @@ __device__ void deep_path(const EnsArgs &a, const EnsReal &g, int r, int member,
     if (t == 0 && !failed) atomicOr(&q[kRhErr], kErrIndex);
     return;   // every member leaves together: the values above are shared
   }
-  // upstream cell counts, top level first (pull form of RB:262-279)
+  // upstream cell counts, top level first, pulled from each entry's
+  // contiguous donors (pull form of RB:262-279)
   for (int l = D; l >= 0; l--) {
     const int ls = lev[l], le = lev[l + 1];
     for (int p = ls + t; p < le; p += T) {
-      const int c = ord[p];
-      uint32_t m = dm[c], s = 1;
-      while (m) {
-        const int j = __builtin_ctz(m);
-        m &= m - 1;
-        s += ct[c + slot_offset(j, W)];
+      const int f = fdon[p], k = ndon[p];
+      uint32_t s = 1;
+      if (f >= le && f + k <= n) {
+        for (int i = 0; i < k; i++) s += cs[f + i];
+      } else if (k > 0) {
+        atomicOr(&q[kRhErr], kErrIndex);
       }
-      ct[c] = s;
+      cs[p] = s;
     }
     group_barrier(G, q + kRhBar, bar, a.ctl);
   }
-  // uplift (RB:285-296) and erosion (RB:306-328), outlets first
+  // uplift (RB:285-296) and erosion (RB:306-328), outlets first; each
+  // entry's receiver is final one level down
   const ErodeConst k = g.ek;
   for (int l = 0; l <= D; l++) {
     const int ls = lev[l], le = lev[l + 1];
@@ -336,12 +361,12 @@ __device__ void deep_path(const EnsArgs &a, const EnsReal &g, int r, int member,
         const bool interior = x >= 2 && x < W - 2 && y >= 2 && y < H - 2;
         hn[c] = interior ? ho[c] + k.ud : ho[c];
       } else {
-        const int rr = rc[c];
+        const int rr = rdir[p] & 7;
         const int nb = c + dir_offset(rr, W);
-        uint32_t cc = ct[c];
-        if (cc > static_cast<uint32_t>(n)) {
+        uint32_t cc = cs[p];
+        if (cc > static_cast<uint32_t>(n) || static_cast<uint32_t>(nb) >= static_cast<uint32_t>(n)) {
           atomicOr(&q[kRhErr], kErrIndex);
-          cc = static_cast<uint32_t>(n);
+          continue;
         }
         hn[c] = newton_update<NMODE>(ho[c] + k.ud, hn[nb], ft[2 * cc + (rr & 1)], k, q);
       }
@@ -531,6 +556,10 @@ struct fs_ensemble {
   uint32_t *cnt = nullptr;
   TilePlan *plan = nullptr;
   int32_t *order = nullptr, *lev = nullptr;
+  uint8_t *rdir = nullptr;                    // level path, stack order
+  int32_t *fdon = nullptr;
+  uint8_t *ndon = nullptr;
+  uint32_t *cnt_s = nullptr;
   double *ftab = nullptr;
   int32_t *rh = nullptr, *hist = nullptr, *ctl = nullptr;
   int32_t *tabA_off = nullptr, *tabA_real = nullptr, *tabE_off = nullptr, *tabE_real = nullptr;
@@ -559,7 +588,9 @@ void free_ensemble(fs_ensemble *e) {
                   static_cast<void *>(e->h[1]), static_cast<void *>(e->rcv),
                   static_cast<void *>(e->lvl), static_cast<void *>(e->dmask),
                   static_cast<void *>(e->cnt), static_cast<void *>(e->plan),
-                  static_cast<void *>(e->order),
+                  static_cast<void *>(e->order), static_cast<void *>(e->rdir),
+                  static_cast<void *>(e->fdon), static_cast<void *>(e->ndon),
+                  static_cast<void *>(e->cnt_s),
                   static_cast<void *>(e->lev), static_cast<void *>(e->ftab),
                   static_cast<void *>(e->rh), static_cast<void *>(e->hist),
                   static_cast<void *>(e->ctl), static_cast<void *>(e->tabA_off),
@@ -642,6 +673,10 @@ int launch_step(fs_ensemble *e) {
   a.cnt = e->cnt;
   a.plan = e->plan;
   a.order = e->order;
+  a.rdir = e->rdir;
+  a.fdon = e->fdon;
+  a.ndon = e->ndon;
+  a.cnt_s = e->cnt_s;
   a.lev = e->lev;
   a.ftab = e->ftab;
   a.rh = e->rh;
@@ -887,6 +922,10 @@ int fs_ensemble_create(const fs_realization *reals, int count, int device, fs_en
   FS_TRY(fs::dev_alloc(&e->cnt, cells));
   FS_TRY(fs::dev_alloc(&e->plan, static_cast<size_t>(tiles_total)));
   FS_TRY(fs::dev_alloc(&e->order, cells));
+  FS_TRY(fs::dev_alloc(&e->rdir, cells));
+  FS_TRY(fs::dev_alloc(&e->fdon, cells));
+  FS_TRY(fs::dev_alloc(&e->ndon, cells));
+  FS_TRY(fs::dev_alloc(&e->cnt_s, cells));
   FS_TRY(fs::dev_alloc(&e->lev, lev_n));
   FS_TRY(fs::dev_alloc(&e->ftab, ftab_n));
   FS_TRY(fs::dev_alloc(&e->rh, static_cast<size_t>(count) * kRH));
@@ -1035,8 +1074,23 @@ int fs_ensemble_get_accum(fs_ensemble *e, int r, double *accum, size_t count) {
   if (!accum) return FS_ERR_INVALID_ARGUMENT;
   int ri;
   FS_TRY(check_index(e, r, count, &ri));
+  const EnsReal &g = e->real[ri];
+  int32_t q[kRH];
+  FS_TRY(download(e, q, e->rh + static_cast<size_t>(ri) * kRH, kRH));
   std::vector<uint32_t> v(count);
-  FS_TRY(download(e, v.data(), e->cnt + e->real[ri].off, count));
+  if (q[kRhPath] == 2) {
+    // the level path keeps counts in stack order: scatter them by cell
+    std::vector<int32_t> ord(count);
+    std::vector<uint32_t> cs(count);
+    FS_TRY(download(e, ord.data(), e->order + g.off, count));
+    FS_TRY(download(e, cs.data(), e->cnt_s + g.off, count));
+    for (size_t p = 0; p < count; p++) {
+      if (static_cast<uint32_t>(ord[p]) >= count) return FS_ERR_INTERNAL;
+      v[static_cast<size_t>(ord[p])] = cs[p];
+    }
+  } else {
+    FS_TRY(download(e, v.data(), e->cnt + g.off, count));
+  }
   const double area = e->spec[ri].params.cell_area;
   for (size_t i = 0; i < count; i++) accum[i] = static_cast<double>(v[i]) * area;
   return FS_OK;

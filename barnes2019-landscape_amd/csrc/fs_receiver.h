@@ -77,26 +77,31 @@ __host__ __device__ inline int receiver_two_div(double hc, const double nb[8], d
   return nc < nd ? nc : nd;
 }
 
-// The group's best direction when it can be told without dividing: the
-// first drop within 2^-50*A of the group's largest drop A is the best
-// direction if it is A itself (earlier drops are too far below A to round to
-// the same quotient).  -1 when that first drop is a smaller near-tie, which
-// only a division can settle.
-__host__ __device__ inline int group_first(const double d[8], int first, double A) {
-  const double window = A - A * 0x1p-50;
-  for (int n = first; n < 8; n += 2)
-    if (d[n] >= window) return d[n] == A ? n : -1;
-  return -1;   // unreachable: A itself is in the window
+// True when the divisors admit the division-free decision of receiver_fast
+// (a property of the model's constants: callers may evaluate it once).
+__host__ __device__ inline bool divisors_in_range(double dc, double dd) {
+  return dc >= 0x1p-60 && dc <= 0x1p60 && dd >= 0x1p-60 && dd <= 0x1p60;
 }
 
 // The reference's direction, decided without division whenever every
-// quotient involved is a normal number and the two groups' slopes are not
-// near-tied.  For drops Ac, Ad and divisors dc, dd in range, P = RN(Ac*dd)
-// and Q = RN(Ad*dc) carry relative errors of at most 2^-53 each, so
-// P > RN(Q*(1 + 2^-48)) gives Ac/dc > (Ad/dd)*(1 + 2^-49), hence
-// RN(Ac/dc) > RN(Ad/dd) strictly: the cardinal group's quotient wins without
-// a tie.  Anything closer (or out of range) goes to receiver_two_div.
-__host__ __device__ inline int receiver_fast(double hc, const double nb[8], double dc, double dd) {
+// quotient involved is a normal number and no two slopes are near-tied.
+//
+// Within a group, RN(d/dr) is monotone in d, so the group's steepest slope
+// is RN(A/dr) for its largest drop A, and a smaller drop can round to the
+// same quotient only when it lies within 2^-52*A of A: drops below the
+// window w = A - A*2^-50 never tie with A.  If the lowest-index drop in
+// [w, A] is A itself, that index is the group's answer; otherwise a
+// near-tie needs the divisions.  Between the groups, for drops Ac, Ad and
+// divisors dc, dd in range, P = RN(Ac*dd) and Q = RN(Ad*dc) carry relative
+// errors of at most 2^-53 each, so P > RN(Q*(1 + 2^-48)) gives
+// Ac/dc > (Ad/dd)*(1 + 2^-49), hence RN(Ac/dc) > RN(Ad/dd) strictly: the
+// cardinal group wins without a tie (and symmetrically).  Anything closer,
+// or out of range, goes to receiver_two_div.
+//
+// Written as straight-line selects: the comparisons combine as lane masks
+// and only the rare fallback branches.
+__host__ __device__ inline int receiver_fast(double hc, const double nb[8], double dc, double dd,
+                                             bool div_ok) {
   double d[8];
   for (int n = 0; n < 8; n++) d[n] = hc - nb[n];
   // Group maxima.  fmax ignores NaN drops, as the reference's strict '>'
@@ -105,24 +110,32 @@ __host__ __device__ inline int receiver_fast(double hc, const double nb[8], doub
   const double Ac = fmax(fmax(d[0], d[2]), fmax(d[4], d[6]));
   const double Ad = fmax(fmax(d[1], d[3]), fmax(d[5], d[7]));
   const bool pc = Ac > 0.0, pd = Ad > 0.0;
-  if (!pc && !pd) return kNoReceiver;
+  const double wc = Ac - Ac * 0x1p-50, wd = Ad - Ad * 0x1p-50;
+  // lowest in-window index of each group, and whether that drop is A
+  const bool c0 = d[0] >= wc, c2 = d[2] >= wc, c4 = d[4] >= wc;
+  const bool d1 = d[1] >= wd, d3 = d[3] >= wd, d5 = d[5] >= wd;
+  const int ic = c0 ? 0 : c2 ? 2 : c4 ? 4 : 6;
+  const int id = d1 ? 1 : d3 ? 3 : d5 ? 5 : 7;
+  const bool fc = (d[0] == Ac) | (!c0 & ((d[2] == Ac) | (!c2 & ((d[4] == Ac) | (!c4 &
+                  (d[6] == Ac))))));
+  const bool fd = (d[1] == Ad) | (!d1 & ((d[3] == Ad) | (!d3 & ((d[5] == Ad) | (!d5 &
+                  (d[7] == Ad))))));
   // quotients of drops in [2^-900, 2^900] by divisors in [2^-60, 2^60] are
   // normal and nonzero: a positive drop is a positive slope
-  const bool okc = !pc || (Ac >= 0x1p-900 && Ac <= 0x1p900);
-  const bool okd = !pd || (Ad >= 0x1p-900 && Ad <= 0x1p900);
-  const bool okr = dc >= 0x1p-60 && dc <= 0x1p60 && dd >= 0x1p-60 && dd <= 0x1p60;
-  if (okc && okd && okr) {
-    const int bc = pc ? group_first(d, 0, Ac) : kNoReceiver;
-    const int bd = pd ? group_first(d, 1, Ad) : kNoReceiver;
-    if (bc >= 0 && bd >= 0) {
-      if (!pd) return bc;
-      if (!pc) return bd;
-      const double P = Ac * dd, Q = Ad * dc;
-      if (P > Q * (1.0 + 0x1p-48)) return bc;
-      if (Q > P * (1.0 + 0x1p-48)) return bd;
-    }
+  const bool rc = (Ac >= 0x1p-900) & (Ac <= 0x1p900);
+  const bool rd = (Ad >= 0x1p-900) & (Ad <= 0x1p900);
+  const double P = Ac * dd, Q = Ad * dc;
+  const bool cw = P > Q * (1.0 + 0x1p-48), dw = Q > P * (1.0 + 0x1p-48);
+  const bool fast = div_ok & (!pc | (rc & fc)) & (!pd | (rd & fd)) & (!(pc & pd) | cw | dw);
+  if (fast) {
+    if (!pc & !pd) return kNoReceiver;
+    return (pc & (!pd | cw)) ? ic : id;
   }
   return receiver_two_div(hc, nb, dc, dd);
+}
+
+__host__ __device__ inline int receiver_fast(double hc, const double nb[8], double dc, double dd) {
+  return receiver_fast(hc, nb, dc, dd, divisors_in_range(dc, dd));
 }
 
 }  // namespace fs

@@ -332,9 +332,12 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
                                              uint8_t *__restrict__ lvl,
                                              TilePlan *__restrict__ plan, int W, int H, int t,
                                              int tiles_x, int32_t *level_hist,
-                                             int32_t *max_level) {
+                                             int32_t *max_level,
+                                             unsigned long long *phase = nullptr) {
   static_assert(BLOCK == kAccBlock, "shared layout is sized for kAccBlock");
   uint32_t *w_s = sh.w_s;
+  unsigned long long *const ph = phase_row(phase);
+  const uint64_t t0 = ph ? wall_clock64() : 0;
   const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
   const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
   if (threadIdx.x == 0) {
@@ -352,6 +355,7 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
   }
   if (!done) load_region_words<BLOCK>(rcv, w_s, rx0, ry0, W, H);
   __syncthreads();
+  const uint64_t t1 = ph ? wall_clock64() : 0;
   // Every region cell walks its receiver chain and adds 1 to each cell it
   // passes, itself included; the thread's chains advance together, one cell
   // per round.  A walker stops at an outlet (its level is the round it got
@@ -425,6 +429,11 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) deepest = max(deepest, __shfl_xor(deepest, d));
     if ((threadIdx.x & (kWave - 1)) == 0 && deepest > 0) atomicMax(&sh.maxlev, deepest);
+  }
+  uint64_t t2 = 0;
+  if (ph) {
+    __syncthreads();
+    t2 = wall_clock64();
   }
   mark_halo_chains<BLOCK>(w_s, sh.mark, sh.hlist, &sh.nlist);
   __syncthreads();
@@ -500,6 +509,11 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
     }
     if (threadIdx.x == 0) plan->nhalo = static_cast<int32_t>(nhalo);
   }
+  uint64_t t3 = 0;
+  if (ph) {
+    __syncthreads();
+    t3 = wall_clock64();
+  }
   for (int it = 0; it < kTileCells / BLOCK; it++) {
     const int i = it * BLOCK + threadIdx.x;
     const int tx = i % kTile, ty = i / kTile;
@@ -509,20 +523,35 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
     cnt[gi] = w_s[(ty + kHalo) * kReg + tx + kHalo] & kAccCount;
     lvl[gi] = sh.lv_t[i];
   }
+  if (ph) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint64_t t4 = wall_clock64();
+      atomicAdd(&ph[kPhAccLoad], t1 - t0);
+      atomicAdd(&ph[kPhAccWalk], t2 - t1);
+      atomicAdd(&ph[kPhAccPlan], t3 - t2);
+      atomicAdd(&ph[kPhAccStore], t4 - t3);
+    }
+  }
   return tile_over;
 }
 
+// kTimed: this launch records phase ticks (the host runs it on a sample of
+// steps; the untimed instantiation carries no timing code at all).
+template <bool kTimed>
 __global__ __launch_bounds__(kAccBlock) void k_tile_acc(const uint8_t *__restrict__ rcv,
                                                         uint32_t *__restrict__ cnt,
                                                         uint8_t *__restrict__ lvl,
                                                         TilePlan *__restrict__ plan, int32_t *hdr,
-                                                        int W, int H, int tiles_x, int ntiles) {
+                                                        int W, int H, int tiles_x, int ntiles,
+                                                        unsigned long long *phase) {
   __shared__ AccShared sh;
   const int t = xcd_tile(blockIdx.x, ntiles);
   // with the deep halo, hdr[0] (cleared by k_fill_counts) receives the
   // deepest tile-cell level, so the host can step back to the 16-cell halo
   const int over = tile_acc_body<kAccBlock>(sh, rcv, cnt, lvl, plan + t, W, H, t, tiles_x, nullptr,
-                                            kHalo > 16 ? hdr : nullptr);
+                                            kHalo > 16 ? hdr : nullptr,
+                                            kTimed ? phase : nullptr);
   if (threadIdx.x == 0 && over) atomicAdd(&hdr[3], 1);
 }
 
@@ -566,8 +595,11 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
                                                 const uint32_t *__restrict__ cnt,
                                                 const double *__restrict__ ftab, int W, int H,
                                                 int t, int tiles_x, const ErodeConst &k,
-                                                int32_t *hdr) {
+                                                int32_t *hdr,
+                                                unsigned long long *phase = nullptr) {
   constexpr int kWaves = kEroBlock / kWave;
+  unsigned long long *const ph = phase_row(phase);
+  const uint64_t t0 = ph ? wall_clock64() : 0;
   constexpr int kKeys = kHalo + 1;
   constexpr int kTilePer = kTileCells / kEroBlock;
   constexpr int kSlotPer = (kSlots + kEroBlock - 1) / kEroBlock;
@@ -697,48 +729,59 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
       list[e] = 0xFFFF;
   __syncthreads();
   const int nchunks = loff[kKeys] / kWave, root_chunks = loff[1] / kWave;
-  for (int ch = wave; ch < nchunks; ch += kWaves) {
+  const uint64_t t1 = ph ? wall_clock64() : 0;
+  // level 0 first: uplift of the interior outlets; rings 0 and 1 stay fixed
+  int ch = wave;
+  for (; ch < root_chunks; ch += kWaves) {
     const int sl = list[ch * kWave + lane];
     if (sl == 0xFFFF) continue;
-    if (ch < root_chunks) {
-      // level 0: uplift of interior outlets; rings 0 and 1 stay fixed
-      int gx, gy;
-      if (sl < kTileCells) {
-        gx = x0 + sl % kTile;
-        gy = y0 + sl / kTile;
-      } else {
-        const int i = sh.hreg[sl - kTileCells];
-        gx = rx0 + i % kReg;
-        gy = ry0 + i / kReg;
-      }
-      if (gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2) hs[sl] += k.ud;
+    int gx, gy;
+    if (sl < kTileCells) {
+      gx = x0 + sl % kTile;
+      gy = y0 + sl / kTile;
     } else {
-      const int rs = rs_s[sl];
-      const int nb = rs & kPlanSlot;
-      if (nb < nslots) {                      // always, for a cell above level 0
-        // the cell's own inputs do not depend on its receiver: read them
-        // before the acquire wait, which later loads may not pass
-        const uint32_t a = c_s[sl];
-        const int ti = 2 * static_cast<int>(a) + (rs >> 15);
-        const double fact = a < kEroFactCache ? ft_s[ti] : ftab[ti];
-        const double h0 = hs[sl] + k.ud;
-        int spins = 0;
-        while (__hip_atomic_load(&d_s[nb], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) !=
-               kDone) {
-          if (++spins > (1 << 22)) {
-            atomicOr(&hdr[2], kErrLookback);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
+      const int i = sh.hreg[sl - kTileCells];
+      gx = rx0 + i % kReg;
+      gy = ry0 + i / kReg;
+    }
+    if (gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2) hs[sl] += k.ud;
+    __hip_atomic_store(&d_s[sl], kDone, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  uint64_t tm = 0;
+  if (ph) {
+    __syncthreads();
+    tm = wall_clock64();
+  }
+  // levels 1 and up, each cell as soon as its receiver is done
+  for (; ch < nchunks; ch += kWaves) {
+    const int sl = list[ch * kWave + lane];
+    if (sl == 0xFFFF) continue;
+    const int rs = rs_s[sl];
+    const int nb = rs & kPlanSlot;
+    if (nb < nslots) {                      // always, for a cell above level 0
+      // the cell's own inputs do not depend on its receiver: read them
+      // before the acquire wait, which later loads may not pass
+      const uint32_t a = c_s[sl];
+      const int ti = 2 * static_cast<int>(a) + (rs >> 15);
+      const double fact = a < kEroFactCache ? ft_s[ti] : ftab[ti];
+      const double h0 = hs[sl] + k.ud;
+      int spins = 0;
+      while (__hip_atomic_load(&d_s[nb], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) !=
+             kDone) {
+        if (++spins > (1 << 22)) {
+          atomicOr(&hdr[2], kErrLookback);
+          break;
         }
-        hs[sl] = newton_update<NMODE>(h0, hs[nb], fact, k, hdr);
-      } else {
-        atomicOr(&hdr[2], kErrIndex);
+        __builtin_amdgcn_s_sleep(1);
       }
+      hs[sl] = newton_update<NMODE>(h0, hs[nb], fact, k, hdr);
+    } else {
+      atomicOr(&hdr[2], kErrIndex);
     }
     __hip_atomic_store(&d_s[sl], kDone, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   __syncthreads();
+  const uint64_t t2 = ph ? wall_clock64() : 0;
 #pragma unroll
   for (int j = 0; j < kTilePer; j++) {
     const int tj = j * kEroBlock + threadIdx.x;
@@ -746,20 +789,31 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
     if (gx >= W || gy >= H) continue;
     h_new[static_cast<size_t>(gy) * W + gx] = hs[tj];
   }
+  if (ph) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint64_t t3 = wall_clock64();
+      atomicAdd(&ph[kPhEroBucket], t1 - t0);
+      atomicAdd(&ph[kPhEroRoots], tm - t1);
+      atomicAdd(&ph[kPhEroLoop], t2 - tm);
+      atomicAdd(&ph[kPhEroStore], t3 - t2);
+    }
+  }
 }
 
-template <int NMODE>
+template <int NMODE, bool kTimed>
 __global__ __launch_bounds__(kEroBlock) void k_tile_erode(
     const double *__restrict__ h_old, double *__restrict__ h_new,
     const TilePlan *__restrict__ plan, const uint8_t *__restrict__ lvl,
     const uint32_t *__restrict__ cnt, const double *__restrict__ ftab, int W, int H,
-    int tiles_x, int ntiles, ErodeConst k, int32_t *hdr) {
+    int tiles_x, int ntiles, ErodeConst k, int32_t *hdr, unsigned long long *phase) {
   __shared__ EroShared sh;
   // launched right behind the accumulation, before the host has read its
   // outcome: when a tile was too deep (hdr[3], the plans are incomplete)
   // the whole grid does nothing
   if (hdr[3] != 0) return;
   const int t = xcd_tile(blockIdx.x, ntiles);
-  tile_erode_body<NMODE>(sh, h_old, h_new, plan + t, lvl, cnt, ftab, W, H, t, tiles_x, k, hdr);
+  tile_erode_body<NMODE>(sh, h_old, h_new, plan + t, lvl, cnt, ftab, W, H, t, tiles_x, k, hdr,
+                         kTimed ? phase : nullptr);
 }
 

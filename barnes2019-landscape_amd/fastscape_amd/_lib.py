@@ -34,7 +34,8 @@ EXPORTED = [
     "fs_step_noalloc",
     "fs_get_h", "fs_set_h", "fs_get_timers", "fs_reset_timers", "fs_get_receivers",
     "fs_get_donors", "fs_get_stack", "fs_get_levels", "fs_get_accum", "fs_device_h",
-    "fs_stream", "fs_synchronize", "fs_set_step_path", "fs_path_stats", "fs_tile_stats", "fs_set_kernel_timing", "fs_kernel_stats",
+    "fs_stream", "fs_synchronize", "fs_fill_depressions", "fs_set_depression_fill",
+    "fs_set_step_path", "fs_path_stats", "fs_tile_stats", "fs_set_kernel_timing", "fs_kernel_stats",
     "fs_kernel_name", "fs_calibrate_copy", "fs_ensemble_create", "fs_ensemble_destroy",
     "fs_ensemble_size", "fs_ensemble_run", "fs_ensemble_step", "fs_ensemble_get_h",
     "fs_ensemble_set_h", "fs_ensemble_get_receivers", "fs_ensemble_get_accum",
@@ -120,6 +121,8 @@ def _lib():
         "fs_device_h": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
         "fs_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
         "fs_synchronize": (ctypes.c_int, [_vp]),
+        "fs_fill_depressions": (ctypes.c_int, [_vp]),
+        "fs_set_depression_fill": (ctypes.c_int, [_vp, ctypes.c_int]),
         "fs_set_step_path": (ctypes.c_int, [_vp, ctypes.c_int]),
         "fs_path_stats": (ctypes.c_int, [_vp, _u64p, _u64p]),
         "fs_tile_stats": (ctypes.c_int, [_vp, _u64p, _u64p]),
@@ -342,6 +345,15 @@ class FastScape:
         p = _vp()
         _check(_lib().fs_device_h(self._m, ctypes.byref(p)), "fs_device_h")
         return p.value
+
+    def fill_depressions(self):
+        """Fill the current field's depressions once (Priority-Flood+epsilon)."""
+        _check(_lib().fs_fill_depressions(self._m), "fs_fill_depressions")
+
+    def set_depression_fill(self, enable):
+        """Fill depressions before the receivers of every following step."""
+        _check(_lib().fs_set_depression_fill(self._m, 1 if enable else 0),
+               "fs_set_depression_fill")
 
     def set_path(self, path):
         """PATH_AUTO (tile path when levels allow) or PATH_LEVELS."""

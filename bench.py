@@ -92,34 +92,98 @@ def parse():
                     help="steps timed per kernel with HIP events for the roofline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-size", type=int, default=10000)
-    ap.add_argument("--cpu-sample-run", type=int, default=1,
+    ap.add_argument("--cpu-sample-run", type=int, default=2,
                     help="run(n) of the CPU sample, i.e. n+1 time-steps")
+    ap.add_argument("--cpu-reps", type=int, default=3,
+                    help="repetitions of the CPU sample (the median is reported)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a 1 GiB plain copy kernel (counter calibration)")
     return ap.parse_args()
 
 
-def cpu_baseline(size, nrun):
-    """Reference RB+P (fastscape_RB+P.cpp, OpenMP) built from the reference
-    sources under oracle/_ref; falls back to the oracle's own RB+P
-    restatement when the reference build is absent."""
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+def host_cpu():
+    """CPU model, physical cores (distinct (package, core) pairs) and the CPUs
+    this process may run on."""
+    model, cores = "unknown", set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model == "unknown":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return model, (len(cores) or None), allowed
+
+
+def _timed_runs(cmd, env, reps):
+    secs = []
+    for _ in range(reps):
+        out = subprocess.run(cmd, check=True, capture_output=True, text=True, env=env).stdout
+        secs.append(float(re.search(r"m run_seconds = ([0-9.eE+-]+)", out).group(1)))
+    return secs
+
+
+def cpu_baseline(size, nrun, reps=3, serial_nrun=0, serial_reps=3, serial=True):
+    """The reference's RB+P (fastscape_RB+P.cpp, OpenMP) built from the
+    reference sources under oracle/_ref, timed as tests.sh does
+    (tests/tests.sh:45-48): `reps` repetitions of run(nrun) on the benchmark
+    preset, median taken.  Threads: OMP_NUM_THREADS when set (the CPUs this
+    host grants per GPU), else every CPU this process may use.  With
+    `serial`, also the serial RB (fastscape_RB.cpp) on one thread, median of
+    `serial_reps` runs of run(serial_nrun).  Falls back to the oracle's own
+    restatements (kind "port") when the reference build is absent."""
+    import statistics
+    model, physical, allowed = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or allowed
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-    ref = os.path.join(ROOT, "oracle", "_ref", "ref_rbp")
-    if os.path.exists(ref):
-        cmd = [ref, "run", str(size), str(size), str(nrun), "123", "1", "-"]
-        kind = "reference"
-    else:
-        cmd = [os.path.join(ROOT, "oracle", "oracle_cli"), str(size), str(size), str(nrun), "0",
-               "1", "0", "1", "-"]
-        kind = "port"
-    out = subprocess.run(cmd, check=True, capture_output=True, text=True, env=env).stdout
-    secs = float(re.search(r"m run_seconds = ([0-9.eE+-]+)", out).group(1))
+    ref_dir = os.path.join(ROOT, "oracle", "_ref")
+    kind = "reference" if os.path.exists(os.path.join(ref_dir, "ref_rbp")) else "port"
+
+    def cmd(par, n):
+        if kind == "reference":
+            exe = os.path.join(ref_dir, "ref_rbp" if par else "ref_rb")
+            return [exe, "run", str(size), str(size), str(n), "123", "1", "-"]
+        return [os.path.join(ROOT, "oracle", "oracle_cli"), str(size), str(size), str(n), "0",
+                "1", "0", "1" if par else "0", "-"]
+
+    secs = _timed_runs(cmd(True, nrun), env, reps)
+    med = statistics.median(secs)
     cells = size * size * (nrun + 1)
-    return {"value": cells / secs, "unit": "cell-steps/s", "cores": threads, "kind": kind,
-            "sample": "%dx%d benchmark preset, run(%d) = %d time-steps, %.1f s, %s" % (
-                size, size, nrun, nrun + 1, secs,
-                "fastscape_RB+P.cpp" if kind == "reference" else "oracle RB+P restatement")}
+    line = {"value": cells / med, "unit": "cell-steps/s", "cores": threads, "kind": kind,
+            "sample": "%dx%d benchmark preset, run(%d) = %d time-steps, median of %d: %s s, %s"
+                      % (size, size, nrun, nrun + 1, reps,
+                         "/".join("%.2f" % x for x in secs),
+                         "fastscape_RB+P.cpp" if kind == "reference"
+                         else "oracle RB+P restatement"),
+            "cpu_model": model, "cores_physical": physical, "cpus_allowed": allowed,
+            "repetitions": reps, "seconds": secs}
+    if serial:
+        one = dict(env, OMP_NUM_THREADS="1")
+        ssecs = _timed_runs(cmd(False, serial_nrun), one, serial_reps)
+        smed = statistics.median(ssecs)
+        line["serial_rb_1_thread"] = {
+            "value": size * size * (serial_nrun + 1) / smed, "unit": "cell-steps/s",
+            "cores": 1, "seconds": ssecs,
+            "sample": "%dx%d benchmark preset, run(%d), median of %d, %s"
+                      % (size, size, serial_nrun, serial_reps,
+                         "fastscape_RB.cpp" if kind == "reference" else "oracle RB restatement")}
+    return line
 
 
 def bench_ensemble(args, fa):
@@ -159,7 +223,7 @@ def bench_ensemble(args, fa):
     ens.close()
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(1024, 29)
+        cpu = cpu_baseline(1024, 29, reps=3, serial_nrun=9)
     tiles = sum(t for t, _ in paths)
     lvls = sum(l for _, l in paths)
     line = {"metric": "ensemble cell-steps/s, FP64 (one persistent launch per step)",
@@ -237,12 +301,32 @@ def bench_inversion(args, fa):
     prop.close()
     cpu = None
     if not args.no_cpu_baseline:
-        one = cpu_baseline(side, nstep - 1)
+        one = cpu_baseline(side, nstep - 1, reps=1, serial=False)
         per_run = side * side * nstep / one["value"]
         cpu = {"value": per_run * len(cands), "unit": "s (exact-only search on the CPU)",
                "cores": one["cores"], "kind": one["kind"],
                "sample": "one %dx%d run(%d) of the benchmark preset, %.2f s, times %d candidates"
                          % (side, side, nstep - 1, per_run, len(cands))}
+    # one-off cost of the proposal: exact training runs and training; what it
+    # buys per target: t_exact - t_prop
+    train_s = st["train_ms"] / 1e3
+    one_off = gen_wall + train_s
+    gain = t_exact - t_prop
+    break_even = one_off / gain if gain > 0 else None
+    end_to_end = {
+        "one_off_seconds": one_off,
+        "training_set_wall_seconds": gen_wall,
+        "training_seconds": train_s,
+        "speedup_per_target_excluding_one_off": t_exact / t_prop,
+        "speedup_single_target_including_one_off": t_exact / (one_off + t_prop),
+        "break_even_targets": break_even,
+        "speedup_amortised_over_100_targets": 100 * t_exact / (one_off + 100 * t_prop),
+    }
+    log("end to end: one-off %.1f s (training set %.1f s + training %.1f s); per target %.2f s vs "
+        "%.2f s exact-only; single target %.3fx; break-even after %s targets"
+        % (one_off, gen_wall, train_s, t_prop, t_exact, end_to_end[
+            "speedup_single_target_including_one_off"],
+           "%.1f" % break_even if break_even else "no"))
     line = {"metric": "exact-vs-proposal inversion wall time", "value": t_prop, "unit": "s",
             "n_gpus": 1, "steps": nstep, "warmup": 0, "ms_per_step": None,
             "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
@@ -254,6 +338,7 @@ def bench_inversion(args, fa):
                                       args.train_runs, args.epochs),
                        "hidden_params": hidden[0].tolist(), "terrain_seed": seed},
             "exact_only_seconds": t_exact, "speedup_vs_exact_only": t_exact / t_prop,
+            "end_to_end": end_to_end,
             "result": {"exact_only_best": cands[best_e].tolist(),
                        "proposal_best": cands[best_p].tolist(),
                        "same_candidate": bool(best_p == best_e),
@@ -377,7 +462,8 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline and args.preset == "benchmark":   # timed on that preset
-        cpu = cpu_baseline(args.cpu_sample_size, args.cpu_sample_run)
+        cpu = cpu_baseline(args.cpu_sample_size, args.cpu_sample_run, reps=args.cpu_reps,
+                           serial_nrun=0, serial_reps=args.cpu_reps)
 
     line = {"metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": secs * 1e3 / args.steps,

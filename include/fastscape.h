@@ -122,7 +122,7 @@ int fs_step(fs_model *m, int iterations);
  * place by `iterations` time-steps on `stream` (a hipStream_t; NULL = the
  * null stream); same results, bit for bit, as fs_create_from_h + fs_step.
  * Every work array comes from one device array built into the library,
- * sized for at most FS_NOALLOC_MAX_CELLS cells (about 62 MB, held on each
+ * sized for at most FS_NOALLOC_MAX_CELLS cells (about 73 MB, held on each
  * GPU the library runs on until the process ends); larger grids return
  * FS_ERR_DIMENSIONS.  Grids of more than 512 64x64 tiles take the level
  * path.  All calls on one GPU share that array, which keeps the last call's
@@ -173,6 +173,21 @@ int fs_path_stats(fs_model *m, uint64_t *tile_steps, uint64_t *level_steps);
 /* Tile-path steps split by halo (16-cell, 32-cell).  No reference
  * counterpart. */
 int fs_tile_stats(fs_model *m, uint64_t *halo16_steps, uint64_t *halo32_steps);
+
+/* ---- depression filling ----------------------------------------------------
+ * No counterpart in the reference's code: its README claims depression
+ * handling (README.md:35-37), but pits stay NO_FLOW there and never erode.
+ * The fill raises every interior cell to the Priority-Flood+epsilon surface
+ * (Barnes, Lehman & Mulla 2014): the lowest surface at or above h on which
+ * every interior cell has a strictly lower D8 neighbour (one ulp, by
+ * nextafter), rings 0 and 1 unchanged -- the same bits as a priority-flood
+ * on the CPU.  fs_fill_depressions fills the current field once;
+ * fs_set_depression_fill(m, 1) fills before the receivers of every
+ * following time-step, so every interior cell drains to the rings (off by
+ * default: the reference's behaviour).  Not available to models stepped by
+ * fs_step_noalloc. */
+int fs_fill_depressions(fs_model *m);
+int fs_set_depression_fill(fs_model *m, int enable);
 
 /* ---- state of the last completed time-step (parity checks) -------------- */
 /* After a tile-path step, levels[], the stack and the donor lists are

@@ -371,6 +371,77 @@ int fso_model_run(fso_model *m, int nstep, int powmode, int parallel_erosion) {
 /* PrintDEM (RB:20-40): the inner (W-2)x(H-2) cells, default ostream
  * formatting of a double (printf "%.6g"), each value followed by a space,
  * each row by a newline. */
+/* ---- depression filling: Priority-Flood+epsilon (see the header) ---- */
+typedef struct { double v; int c; } fso_heap_item;
+
+/* Binary min-heap on (value, cell index): ties pop the lower index first, so
+ * the order is deterministic (the filled values do not depend on it). */
+static int fso_heap_less(fso_heap_item a, fso_heap_item b) {
+  return a.v < b.v || (a.v == b.v && a.c < b.c);
+}
+
+static void fso_heap_push(fso_heap_item *hp, int *n, fso_heap_item x) {
+  int i = (*n)++;
+  while (i > 0) {
+    const int pa = (i - 1) / 2;
+    if (!fso_heap_less(x, hp[pa])) break;
+    hp[i] = hp[pa];
+    i = pa;
+  }
+  hp[i] = x;
+}
+
+static fso_heap_item fso_heap_pop(fso_heap_item *hp, int *n) {
+  const fso_heap_item top = hp[0];
+  const fso_heap_item last = hp[--(*n)];
+  int i = 0;
+  for (;;) {
+    int ch = 2 * i + 1;
+    if (ch >= *n) break;
+    if (ch + 1 < *n && fso_heap_less(hp[ch + 1], hp[ch])) ch++;
+    if (!fso_heap_less(hp[ch], last)) break;
+    hp[i] = hp[ch];
+    i = ch;
+  }
+  if (*n > 0) hp[i] = last;
+  return top;
+}
+
+void fso_fill_depressions(double *h, int W, int H) {
+  const int N = W * H;
+  unsigned char *closed = (unsigned char *)calloc((size_t)N, 1);
+  fso_heap_item *hp = (fso_heap_item *)malloc(sizeof(fso_heap_item) * (size_t)N);
+  int n = 0;
+  int nshift[8];
+  fso_nshift(W, nshift);
+  if (!closed || !hp) abort();
+  /* seeds: rings 0 and 1 keep their elevations */
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++)
+      if (x < 2 || y < 2 || x >= W - 2 || y >= H - 2) {
+        const int c = y * W + x;
+        closed[c] = 1;
+        fso_heap_item it = {h[c], c};
+        fso_heap_push(hp, &n, it);
+      }
+  while (n > 0) {
+    const fso_heap_item it = fso_heap_pop(hp, &n);
+    const int c = it.c, x = c % W, y = c / W;
+    if (x < 1 || y < 1 || x >= W - 1 || y >= H - 1) continue;   /* ring 0: no interior neighbour */
+    const double up = nextafter(h[c], INFINITY);
+    for (int k = 0; k < 8; k++) {
+      const int nb = c + nshift[k];
+      if (closed[nb]) continue;
+      closed[nb] = 1;
+      h[nb] = fmax(h[nb], up);   /* fmax: a NaN elevation takes the spill value */
+      fso_heap_item nx = {h[nb], nb};
+      fso_heap_push(hp, &n, nx);
+    }
+  }
+  free(hp);
+  free(closed);
+}
+
 int fso_write_dem(const char *filename, const double *h, int W, int H) {
   FILE *f = fopen(filename, "w");
   if (!f) return -1;

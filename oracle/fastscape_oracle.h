@@ -95,6 +95,17 @@ int  fso_model_step(fso_model *m, int powmode, int parallel_erosion);
 /* run(nstep): nstep+1 time-steps (RB:366), timers accumulated. */
 int  fso_model_run(fso_model *m, int nstep, int powmode, int parallel_erosion);
 
+/* ---- depression filling (an extension: the reference has none) ----
+ * Priority-Flood+epsilon (Barnes, Lehman & Mulla 2014, "Priority-Flood: An
+ * optimal depression-filling and watershed-labeling algorithm for digital
+ * elevation models", Computers & Geosciences 62:117-127, Algorithm 3 with a
+ * single priority queue): rings 0 and 1 are the seeds and keep their
+ * elevations; cells leave a min-heap in increasing order of their filled
+ * value, and each unvisited D8 neighbour n of a popped cell c receives
+ * max(h[n], nextafter(F[c], +inf)).  The reference's README claims
+ * depression handling (README.md:35-37) but its code has none. */
+void fso_fill_depressions(double *h, int W, int H);
+
 /* ArcGIS ASCII grid writer, byte-compatible with PrintDEM (RB:20-40). */
 int  fso_write_dem(const char *filename, const double *h, int W, int H);
 

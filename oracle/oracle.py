@@ -62,6 +62,7 @@ def lib():
         L.fso_erode_levels_omp.argtypes = [_ip, _ip, ctypes.c_int, _ip, _dp, _dp, ctypes.c_int,
                                            ctypes.c_int, ctypes.POINTER(Params), ctypes.c_int]
         L.fso_write_dem.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int]
+        L.fso_fill_depressions.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -196,6 +197,15 @@ def run(W, H, nstep, p, engine_seed=0, powmode=POW_LIBM, h=None):
     h = init_terrain(W, H, engine_seed) if h is None else np.array(h, dtype=np.float64).ravel()
     for _ in range(nstep + 1):
         h = step(h, W, H, p, powmode)["h"]
+    return h
+
+
+def fill_depressions(h, W, H):
+    """Priority-Flood+epsilon (Barnes, Lehman & Mulla 2014) on a copy of h:
+    the depression-filled surface, rings 0 and 1 unchanged.  No reference
+    counterpart (the reference has no depression handling)."""
+    h = np.array(h, dtype=np.float64, copy=True).ravel()
+    lib().fso_fill_depressions(_d(h), W, H)
     return h
 
 

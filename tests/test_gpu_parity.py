@@ -26,6 +26,7 @@ from conftest import (FINAL_FIXTURES, ROOT, STAGE_FIXTURES, cone_terrain, golden
                       load_golden)
 
 import fastscape_amd as fa
+import tabular
 
 pytestmark = pytest.mark.gpu
 REL_TOL_N2 = 1e-12
@@ -288,6 +289,8 @@ def test_cli_dropin(oracle, tmp_path):
     assert lines[2].startswith("h git_hash") and lines[3] == "m Random seed = 123"
     assert "p Step = 0" in lines and "p Step = 20" in lines and "p Step = 30" not in lines
     assert sum(1 for l in lines if l.startswith("t ")) == 9
+    # the results pipeline's columns (d.make_tabular:27-44)
+    tabular.check_cli_row(tabular.row(lines), 31)
     h = oracle.run(101, 101, 30, oracle_params(oracle, 1), 0)
     oracle.write_dem(str(tmp_path / "o.dem"), h, 101, 101)
     assert out.read_bytes() == (tmp_path / "o.dem").read_bytes()
@@ -378,6 +381,34 @@ def test_kernel_and_stage_timing_on_both_paths(oracle):
         assert st["k_tile_acc"][1] == 0
         t = a.timers()
         assert all(v >= 0 for v in t.values()) and max(t.values()) > 0, t
+
+
+def test_stage_timers_cover_every_stage_on_both_paths():
+    """The reference's seven stage timers (RB:380-387) are all non-zero on the
+    tile path too: its two kernels' event time is split over donors, order,
+    accumulation, uplift and erosion by the phase ticks of sampled
+    workgroups.  The split adds up to the kernels' time and stays within the
+    wall time of the run."""
+    note("stage timers 1000^2")
+    W = H = 1000
+    stages = fa.TIMER_NAMES[1:7]
+    with fa.FastScape(W, H, gpu_params(1), engine_seed=0) as m:
+        m.reset_timers()
+        m.step(20)
+        assert m.path_stats() == {"tile": 20, "levels": 0}
+        t = m.timers()
+        note("tile path timers %s" % t)
+        assert all(t[k] > 0 for k in stages), t
+        assert sum(t[k] for k in stages) <= t["Overall"]
+        # accumulation and erosion dominate, as in the reference
+        assert t["Step7: Erosion"] > t["Step6: Uplift"]
+        m.reset_timers()
+        m.set_path(fa.PATH_LEVELS)
+        m.step(5)
+        t = m.timers()
+        note("level path timers %s" % t)
+        assert all(t[k] > 0 for k in stages), t
+        assert sum(t[k] for k in stages) <= t["Overall"]
 
 
 @pytest.mark.parametrize("W,H", [(300, 260), (130, 70)])
