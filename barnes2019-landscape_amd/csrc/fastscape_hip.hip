@@ -57,6 +57,12 @@ namespace tile32 {
 }  // namespace tile32
 }  // namespace
 
+#include "fs_deep.h"
+
+namespace {
+constexpr int kGraphRetry = 64;   // graph steps between attempts of the 32-cell halo
+}  // namespace
+
 // ===========================================================================
 // host side
 // ===========================================================================
@@ -134,6 +140,19 @@ struct fs_model {
   // that allocates nothing): no lazy allocations, no events, no side stream
   bool owns_memory = true;
   std::vector<Span> spans;  // level plan of the current sweep
+
+  // the tile-graph path (allocated on first use)
+  uint32_t *dinfo = nullptr;       // per cell: tail, hops to it, exit tail
+  uint32_t *link = nullptr;        // per tile-border cell: node of its tail
+  uint64_t *node_word = nullptr;   // per node: arrivals and count
+  uint32_t *node_kids = nullptr, *node_parent = nullptr, *node_out = nullptr;
+  int32_t *node_cell = nullptr, *nexit = nullptr;
+  uint16_t *node_tier = nullptr;
+  int32_t *tstat = nullptr, *tlist = nullptr;   // per tier: tiles, their list
+  int32_t *tstat_h = nullptr;                   // pinned copy of tstat
+  uint64_t graph_steps = 0;
+  int32_t graph_tiers = 0;          // erosion launches of the last graph step
+  int graph_run = 0;                // graph steps since the last tile attempt
 
   bool ktiming = false;
   std::vector<KTimed> kpending;
@@ -286,8 +305,14 @@ void free_model(fs_model *m) {
                     static_cast<void *>(m->ftab), static_cast<void *>(m->status),
                     static_cast<void *>(m->ticket), static_cast<void *>(m->plan),
                     static_cast<void *>(m->plan32), static_cast<void *>(m->phase),
-                    static_cast<void *>(m->fill_flag)})
+                    static_cast<void *>(m->fill_flag), static_cast<void *>(m->dinfo),
+                    static_cast<void *>(m->link), static_cast<void *>(m->node_word),
+                    static_cast<void *>(m->node_kids), static_cast<void *>(m->node_parent),
+                    static_cast<void *>(m->node_out), static_cast<void *>(m->node_cell),
+                    static_cast<void *>(m->nexit), static_cast<void *>(m->node_tier),
+                    static_cast<void *>(m->tstat), static_cast<void *>(m->tlist)})
     if (ptr) (void)hipFree(ptr);
+  if (m->tstat_h) (void)hipHostFree(m->tstat_h);
   if (m->hdr_h) (void)hipHostFree(m->hdr_h);
   if (m->levels_h) (void)hipHostFree(m->levels_h);
   for (auto &set : m->ev)
@@ -752,6 +777,121 @@ int tile_path(fs_model *m, hipEvent_t *ev, bool *done) {
   return FS_OK;
 }
 
+// Buffers of the tile-graph path, on its first step.
+int ensure_graph(fs_model *m) {
+  if (m->dinfo) return FS_OK;
+  if (!m->owns_memory) return FS_ERR_INTERNAL;
+  const size_t N = static_cast<size_t>(m->N);
+  const size_t tiles = static_cast<size_t>((m->W + deep::kSide - 1) / deep::kSide) *
+                       static_cast<size_t>((m->H + deep::kSide - 1) / deep::kSide);
+  const size_t nodes = tiles * deep::kNodeCap;
+  FS_TRY(fs::dev_alloc(&m->dinfo, N));
+  FS_TRY(fs::dev_alloc(&m->link, N));
+  FS_TRY(fs::dev_alloc(&m->node_word, nodes));
+  FS_TRY(fs::dev_alloc(&m->node_kids, nodes));
+  FS_TRY(fs::dev_alloc(&m->node_parent, nodes));
+  FS_TRY(fs::dev_alloc(&m->node_out, nodes));
+  FS_TRY(fs::dev_alloc(&m->node_cell, nodes));
+  FS_TRY(fs::dev_alloc(&m->node_tier, nodes));
+  FS_TRY(fs::dev_alloc(&m->nexit, tiles));
+  FS_TRY(fs::dev_alloc(&m->tstat, deep::kTierWords));
+  FS_TRY(fs::dev_alloc(&m->tlist, static_cast<size_t>(deep::kMaxTier + 1) * tiles));
+  if (hipHostMalloc(reinterpret_cast<void **>(&m->tstat_h), deep::kTierWords * sizeof(int32_t)) !=
+      hipSuccess) {
+    m->tstat_h = nullptr;
+    fs::set_last_error("hipHostMalloc failed");
+    return FS_ERR_OUT_OF_MEMORY;
+  }
+  return FS_OK;
+}
+
+template <int NMODE>
+int launch_graph_erosion(fs_model *m, int tiles, int tiles_x) {
+  hipStream_t s = m->stream;
+  FS_TRY(kt_begin(m, FS_K_GRAPH_FINAL));
+  hipLaunchKernelGGL(deep::k_deep_final<NMODE>, dim3(tiles), dim3(deep::kBlock), 0, s, m->rcv,
+                     m->cnt, m->dinfo, m->link, m->node_out, m->ftab, m->h, m->h_alt, m->W, m->H,
+                     tiles_x, m->ek, m->hdr);
+  FS_TRY(check_launch());
+  FS_TRY(kt_end(m));
+  // the tier counts, read behind the erosion of the root-tailed cells
+  FS_HIP_TRY(hipStreamSynchronize(m->side));
+  const int32_t err = m->tstat_h[0];
+  if (err & deep::kErrTier) return FS_ERR_LEVEL_OVERFLOW;
+  if (err != 0) {
+    fs::set_last_error("tile-graph path: receivers do not form a forest");
+    return FS_ERR_INTERNAL;
+  }
+  const int tmax = m->tstat_h[deep::kMaxTier + 1];
+  for (int t = 1; t <= tmax; t++) {
+    const int n = m->tstat_h[t];
+    if (n <= 0) continue;
+    FS_TRY(kt_begin(m, FS_K_GRAPH_ERODE));
+    hipLaunchKernelGGL(deep::k_deep_erode<NMODE>, dim3(n), dim3(deep::kBlock), 0, s, m->rcv,
+                       m->cnt, m->dinfo, m->node_cell, m->node_tier, m->nexit,
+                       m->tlist + static_cast<size_t>(t) * tiles, m->ftab, m->h, m->h_alt,
+                       m->W, m->H, tiles_x, t, m->ek, m->hdr);
+    FS_TRY(check_launch());
+    FS_TRY(kt_end(m));
+  }
+  m->graph_tiers = tmax;
+  return FS_OK;
+}
+
+// The tile-graph path (fs_deep.h): any depth, no halo.  Receivers are
+// already in rcv.  Returns FS_OK with *done = false, having written nothing
+// of the model's state, when the drainage crosses more than kMaxTier tile
+// borders (the caller runs the level path on the same receivers).
+int graph_path(fs_model *m, hipEvent_t *ev, bool *done) {
+  *done = false;
+  FS_TRY(ensure_graph(m));
+  hipStream_t s = m->stream;
+  const int tiles_x = (m->W + deep::kSide - 1) / deep::kSide;
+  const int tiles = tiles_x * ((m->H + deep::kSide - 1) / deep::kSide);
+  hipLaunchKernelGGL(deep::k_deep_clear, dim3(1), dim3(256), 0, s, m->tstat);
+  FS_TRY(check_launch());
+  FS_TRY(kt_begin(m, FS_K_GRAPH_LOCAL));
+  hipLaunchKernelGGL(deep::k_deep_local, dim3(tiles), dim3(deep::kBlock), 0, s, m->rcv, m->cnt,
+                     m->dinfo, m->link, m->node_word, m->node_kids, m->node_cell, m->nexit,
+                     m->tstat, m->W, m->H, tiles_x);
+  FS_TRY(check_launch());
+  FS_TRY(kt_end(m));
+  FS_TRY(mark(ev[2], s));
+  FS_TRY(kt_begin(m, FS_K_GRAPH_NODES));
+  hipLaunchKernelGGL(deep::k_deep_links, dim3(tiles), dim3(deep::kNodeCap), 0, s, m->rcv,
+                     m->link, m->node_cell, m->nexit, m->node_parent, m->node_kids, m->W);
+  FS_TRY(check_launch());
+  hipLaunchKernelGGL(deep::k_deep_tiers, dim3(tiles), dim3(deep::kNodeCap), 0, s, m->node_parent,
+                     m->nexit, m->node_tier, m->tstat, m->tlist, tiles);
+  FS_TRY(check_launch());
+  FS_TRY(kt_end(m));
+  FS_HIP_TRY(hipEventRecord(m->acc_done, s));
+  FS_HIP_TRY(hipStreamWaitEvent(m->side, m->acc_done, 0));
+  FS_HIP_TRY(hipMemcpyAsync(m->tstat_h, m->tstat, deep::kTierWords * sizeof(int32_t),
+                            hipMemcpyDeviceToHost, m->side));
+  FS_TRY(mark(ev[3], s));
+  FS_TRY(kt_begin(m, FS_K_GRAPH_NODES));
+  hipLaunchKernelGGL(deep::k_deep_sum, dim3(tiles), dim3(deep::kNodeCap), 0, s, m->node_word,
+                     m->node_kids, m->node_parent, m->nexit, m->node_out, m->tstat,
+                     tiles * deep::kNodeCap);
+  FS_TRY(check_launch());
+  FS_TRY(kt_end(m));
+  FS_TRY(mark(ev[4], s));
+  FS_TRY(mark(ev[5], s));
+  int st;
+  if (m->nmode == 1) st = launch_graph_erosion<1>(m, tiles, tiles_x);
+  else if (m->nmode == 2) st = launch_graph_erosion<2>(m, tiles, tiles_x);
+  else st = launch_graph_erosion<0>(m, tiles, tiles_x);
+  if (st == FS_ERR_LEVEL_OVERFLOW) return FS_OK;   // too many tiers: the level path
+  FS_TRY(st);
+  std::swap(m->h, m->h_alt);
+  m->order_current = false;                 // levels, stack, donor masks on request
+  m->cnt_stack = false;
+  m->graph_steps++;
+  *done = true;
+  return FS_OK;
+}
+
 // Depression filling of the current field (an extension; the reference has
 // none): the Priority-Flood+epsilon surface, relaxed tile by tile into the
 // second buffer until a launch changes no cell, then made current.  Each
@@ -794,6 +934,15 @@ int step_once(fs_model *m) {
   FS_TRY(mark(ev[1], m->stream));
   bool done = false;
   if (m->path == FS_PATH_AUTO && m->tile_halo != 0) FS_TRY(tile_path(m, ev, &done));
+  const bool tiled = done;
+  if (!done && (m->path == FS_PATH_TILE_GRAPH || (m->path == FS_PATH_AUTO && m->owns_memory))) {
+    FS_TRY(graph_path(m, ev, &done));
+    if (done && m->path == FS_PATH_AUTO) {
+      // drainage this deep rarely shallows again: try the 32-cell halo
+      // every kGraphRetry graph steps
+      m->tile_halo = ++m->graph_run % kGraphRetry == 0 ? tile32::kHalo : 0;
+    }
+  }
   if (!done) {
     FS_TRY(level_path(m, ev));
     // the next step tries the tile halo this step's depth would have fitted
@@ -802,7 +951,7 @@ int step_once(fs_model *m) {
   }
   FS_TRY(mark(ev[6], m->stream));
   m->ev_pending[m->ev_cur] = ev[0] != nullptr;
-  m->ev_tile[m->ev_cur] = done;
+  m->ev_tile[m->ev_cur] = tiled;
   m->ev_cur ^= 1;
   m->have_step = true;
   return FS_OK;
@@ -1112,7 +1261,9 @@ int fs_set_depression_fill(fs_model *m, int enable) {
 }
 
 int fs_set_step_path(fs_model *m, int path) {
-  if (!m || (path != FS_PATH_AUTO && path != FS_PATH_LEVELS)) return FS_ERR_INVALID_ARGUMENT;
+  if (!m || (path != FS_PATH_AUTO && path != FS_PATH_LEVELS && path != FS_PATH_TILE_GRAPH) ||
+      (path == FS_PATH_TILE_GRAPH && !m->owns_memory))
+    return FS_ERR_INVALID_ARGUMENT;
   m->path = path;
   return FS_OK;
 }
@@ -1121,6 +1272,13 @@ int fs_path_stats(fs_model *m, uint64_t *tile_steps, uint64_t *level_steps) {
   if (!m) return FS_ERR_INVALID_ARGUMENT;
   if (tile_steps) *tile_steps = m->tile_steps;
   if (level_steps) *level_steps = m->level_steps;
+  return FS_OK;
+}
+
+int fs_graph_stats(fs_model *m, uint64_t *graph_steps, int32_t *tiers_last_step) {
+  if (!m) return FS_ERR_INVALID_ARGUMENT;
+  if (graph_steps) *graph_steps = m->graph_steps;
+  if (tiers_last_step) *tiers_last_step = m->graph_tiers;
   return FS_OK;
 }
 
@@ -1135,7 +1293,8 @@ const char *fs_kernel_name(int kernel) {
   static const char *names[FS_NUM_KERNELS] = {"k_receivers", "k_topology",   "k_roots",
                                               "k_bfs",       "k_flowacc",    "k_uplift_roots",
                                               "k_erode",     "k_jump",       "k_tile_acc",
-                                              "k_tile_erode"};
+                                              "k_tile_erode", "k_deep_local", "k_deep_nodes",
+                                              "k_deep_final", "k_deep_erode"};
   if (kernel < 0 || kernel >= FS_NUM_KERNELS) return "unknown";
   return names[kernel];
 }

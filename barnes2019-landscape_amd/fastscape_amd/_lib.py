@@ -22,9 +22,11 @@ TIMER_NAMES = ["Step1: Initialize", "Step2: DetermineReceivers", "Step3: Determi
                "Step4: GenerateOrder", "Step5: FlowAcc", "Step6: Uplift", "Step7: Erosion",
                "Overall"]
 KERNELS = ["k_receivers", "k_topology", "k_roots", "k_bfs", "k_flowacc", "k_uplift_roots",
-           "k_erode", "k_jump", "k_tile_acc", "k_tile_erode"]
+           "k_erode", "k_jump", "k_tile_acc", "k_tile_erode", "k_deep_local", "k_deep_nodes",
+           "k_deep_final", "k_deep_erode"]
 PATH_AUTO = 0
 PATH_LEVELS = 1
+PATH_TILE_GRAPH = 2
 TILE_HALO = 16
 TILE_HALO_DEEP = 32
 
@@ -35,7 +37,7 @@ EXPORTED = [
     "fs_get_h", "fs_set_h", "fs_get_timers", "fs_reset_timers", "fs_get_receivers",
     "fs_get_donors", "fs_get_stack", "fs_get_levels", "fs_get_accum", "fs_device_h",
     "fs_stream", "fs_synchronize", "fs_fill_depressions", "fs_set_depression_fill",
-    "fs_set_step_path", "fs_path_stats", "fs_tile_stats", "fs_set_kernel_timing", "fs_kernel_stats",
+    "fs_set_step_path", "fs_path_stats", "fs_tile_stats", "fs_graph_stats", "fs_set_kernel_timing", "fs_kernel_stats",
     "fs_kernel_name", "fs_calibrate_copy", "fs_ensemble_create", "fs_ensemble_destroy",
     "fs_ensemble_size", "fs_ensemble_run", "fs_ensemble_step", "fs_ensemble_get_h",
     "fs_ensemble_set_h", "fs_ensemble_get_receivers", "fs_ensemble_get_accum",
@@ -126,6 +128,7 @@ def _lib():
         "fs_set_step_path": (ctypes.c_int, [_vp, ctypes.c_int]),
         "fs_path_stats": (ctypes.c_int, [_vp, _u64p, _u64p]),
         "fs_tile_stats": (ctypes.c_int, [_vp, _u64p, _u64p]),
+        "fs_graph_stats": (ctypes.c_int, [_vp, _u64p, ctypes.POINTER(ctypes.c_int32)]),
         "fs_set_kernel_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
         "fs_kernel_stats": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _u64p]),
         "fs_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
@@ -356,14 +359,24 @@ class FastScape:
                "fs_set_depression_fill")
 
     def set_path(self, path):
-        """PATH_AUTO (tile path when levels allow) or PATH_LEVELS."""
+        """PATH_AUTO (halo tiles when levels allow, else the tile graph),
+        PATH_LEVELS or PATH_TILE_GRAPH."""
         _check(_lib().fs_set_step_path(self._m, int(path)), "fs_set_step_path")
 
     def path_stats(self):
+        """Steps taken by each path: {"tile": n, "levels": n, "graph": n}."""
         t = ctypes.c_uint64(0)
         lv = ctypes.c_uint64(0)
         _check(_lib().fs_path_stats(self._m, ctypes.byref(t), ctypes.byref(lv)), "fs_path_stats")
-        return {"tile": t.value, "levels": lv.value}
+        return {"tile": t.value, "levels": lv.value, "graph": self.graph_stats()["steps"]}
+
+    def graph_stats(self):
+        """Tile-graph steps and the erosion launches after the first that the
+        last one needed: {"steps": n, "tiers_last_step": k}."""
+        n = ctypes.c_uint64(0)
+        k = ctypes.c_int32(0)
+        _check(_lib().fs_graph_stats(self._m, ctypes.byref(n), ctypes.byref(k)), "fs_graph_stats")
+        return {"steps": n.value, "tiers_last_step": k.value}
 
     def tile_stats(self):
         """Tile-path steps split by halo: {"halo16": n, "halo32": n}."""

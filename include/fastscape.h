@@ -144,10 +144,12 @@ int fs_set_h(fs_model *m, const double *in, size_t count);
 /* Per-stage times in microseconds, in FS_T_* order (RB:380-387).  Stage
  * times are device time between HIP events bracketing each stage; Overall is
  * host wall time of fs_run / fs_step plus construction.  On the tile path
- * (below) donors, levels and accumulation are one kernel, reported under
- * DetermineDonors; GenerateOrder, FlowAcc and Uplift are then empty (the
- * erosion follows the accumulation directly and uplift is fused into
- * Erosion).  FS_PATH_LEVELS gives the reference's stage split. */
+ * (below) donors, levels and accumulation are one kernel and uplift and
+ * erosion another; their time is split over the stages by phase timestamps
+ * that the kernels record on one step in 32.  On the tile-graph path,
+ * DetermineDonors is the in-tile walks, GenerateOrder the tile graph,
+ * FlowAcc the sums over it, Uplift is fused into Erosion (which also
+ * finishes the counts).  FS_PATH_LEVELS gives the reference's stage split. */
 int fs_get_timers(fs_model *m, uint64_t us[FS_NUM_TIMERS]);
 int fs_reset_timers(fs_model *m);
 
@@ -156,14 +158,21 @@ int fs_reset_timers(fs_model *m);
  * hops from its outlet, accumulation and erosion run per 64x64 tile with a
  * 16-cell halo in LDS (two kernels per step); when every cell is at most
  * FS_TILE_HALO_DEEP hops from it, the same two kernels run with a 32-cell
- * halo; otherwise the step falls back to the level path.  A tile step whose
- * chains need more halo cells than a tile's plan holds (256 with the 16-cell
- * halo, 512 with the 32-cell one) also falls back.  FS_PATH_LEVELS: always
- * the level path -- one launch per level for order, accumulation and
- * erosion, as the reference's stages (RB:366-372).  All give bit-identical
+ * halo; otherwise (or when a tile's chains need more halo cells than its
+ * plan holds: 256 with the 16-cell halo, 512 with the 32-cell one) the step
+ * takes the tile-graph path: per 64x64 tile with no halo, the tiles joined
+ * by the graph of the cells where drainage leaves a tile, one erosion
+ * launch per tile border that the longest drainage path crosses (at most
+ * 1023, beyond which the step takes the level path).  Deep drainage stays
+ * on the tile-graph path and retries the 32-cell halo every 64 steps.
+ * Models over lent buffers (fs_step_noalloc) take the level path instead.
+ * FS_PATH_LEVELS: always the level path -- one launch per level for order,
+ * accumulation and erosion, as the reference's stages (RB:366-372).
+ * FS_PATH_TILE_GRAPH: always the tile-graph path.  All give bit-identical
  * results.  No reference counterpart. */
 #define FS_PATH_AUTO   0
 #define FS_PATH_LEVELS 1
+#define FS_PATH_TILE_GRAPH 2
 #define FS_TILE_HALO   16
 #define FS_TILE_HALO_DEEP 32
 int fs_set_step_path(fs_model *m, int path);
@@ -173,6 +182,10 @@ int fs_path_stats(fs_model *m, uint64_t *tile_steps, uint64_t *level_steps);
 /* Tile-path steps split by halo (16-cell, 32-cell).  No reference
  * counterpart. */
 int fs_tile_stats(fs_model *m, uint64_t *halo16_steps, uint64_t *halo32_steps);
+/* Tile-graph path steps, and the erosion launches after the first that its
+ * last step needed (the most tile borders one drainage path crossed).  No
+ * reference counterpart. */
+int fs_graph_stats(fs_model *m, uint64_t *graph_steps, int32_t *tiers_last_step);
 
 /* ---- depression filling ----------------------------------------------------
  * No counterpart in the reference's code: its README claims depression
@@ -235,7 +248,11 @@ int fs_synchronize(fs_model *m);
 #define FS_K_JUMP      7
 #define FS_K_TILE_ACC  8
 #define FS_K_TILE_ERODE 9
-#define FS_NUM_KERNELS 10
+#define FS_K_GRAPH_LOCAL 10
+#define FS_K_GRAPH_NODES 11
+#define FS_K_GRAPH_FINAL 12
+#define FS_K_GRAPH_ERODE 13
+#define FS_NUM_KERNELS 14
 int fs_set_kernel_timing(fs_model *m, int enable);
 int fs_kernel_stats(fs_model *m, int kernel, double *total_ms, uint64_t *launches);
 const char *fs_kernel_name(int kernel);

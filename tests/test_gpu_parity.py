@@ -213,7 +213,8 @@ def test_deep_tile_path_vs_oracle(oracle, period, offset, halo32):
                     assert 16 < st["levels"].size - 3 <= 32
                     expect = {"halo16": 0, "halo32": 1 if halo32 else 0}
                     assert m.tile_stats() == expect
-                    assert m.path_stats()["levels"] == (0 if halo32 else 1)
+                    ps = m.path_stats()
+                    assert ps["levels"] == 0 and ps["graph"] == (0 if halo32 else 1)
                 compare_step(m, st, "cone %d preset %d step %d" % (period, preset, s))
                 exact = oracle.step(hh, W, H, op, oracle.POW_EXACT)["h"]
                 assert_h(m.getH().ravel(), exact, st["h"], op.neq, "cone h step %d" % s)
@@ -313,10 +314,11 @@ def test_large_grid_invariants(oracle):
 @pytest.mark.parametrize("W,H,preset,steps", [(1000, 800, 1, 30), (130, 70, 1, 40),
                                               (300, 300, 0, 60), (1000, 1000, 1, 500)])
 def test_tile_path_equals_level_path(oracle, W, H, preset, steps):
-    """The tile path and the level path are bit-identical step after step.
+    """The default path and the level path are bit-identical step after step.
     The reference preset deepens past 16 levels within ten steps (32-cell
-    halo) and past 32 soon after (level path); the benchmark preset at 1000^2
-    passes 16 levels after about 350 steps and stays on the tile path."""
+    halo) and past 32 soon after (tile-graph path); the benchmark preset at
+    1000^2 passes 16 levels after about 350 steps and stays on the tile
+    path."""
     note("tile vs levels %dx%d preset %d" % (W, H, preset))
     h0 = oracle.init_terrain(W, H, 0)
     with fa.FastScape(W, H, gpu_params(preset), h=h0) as a, \
@@ -331,14 +333,14 @@ def test_tile_path_equals_level_path(oracle, W, H, preset, steps):
                 np.testing.assert_array_equal(a.levels(), b.levels(), err_msg="levels %d" % s)
                 np.testing.assert_array_equal(a.stack(), b.stack(), err_msg="stack %d" % s)
         pa, pb = a.path_stats(), b.path_stats()
-        assert pb == {"tile": 0, "levels": steps}
-        assert pa["tile"] + pa["levels"] == steps
+        assert pb == {"tile": 0, "levels": steps, "graph": 0}
+        assert pa["levels"] == 0 and pa["tile"] + pa["graph"] == steps
         ta = a.tile_stats()
         assert ta["halo16"] + ta["halo32"] == pa["tile"]
         if preset == 1:
             assert pa["tile"] == steps            # shallow drainage: always tiles
         else:
-            assert pa["levels"] > 0                # deep drainage: falls back
+            assert pa["graph"] > 0                 # deep drainage: the tile graph
         if steps >= 60:
             assert ta["halo16"] > 0 and ta["halo32"] > 0, ta
 
@@ -348,7 +350,7 @@ def test_benchmark_grid_takes_tile_path(oracle):
     W = H = 2000
     with fa.FastScape(W, H, gpu_params(1), engine_seed=0) as m:
         m.step(5)
-        assert m.path_stats() == {"tile": 5, "levels": 0}
+        assert m.path_stats() == {"tile": 5, "levels": 0, "graph": 0}
         h = m.getH().ravel().copy()
         st = oracle.step(h, W, H, oracle_params(oracle, 1))
         m.step(1)
@@ -371,7 +373,7 @@ def test_kernel_and_stage_timing_on_both_paths(oracle):
         st = a.kernel_stats()
         for k in ("k_receivers", "k_tile_acc", "k_tile_erode"):
             assert st[k][1] == 6 and st[k][0] > 0, (k, st[k])
-        assert a.path_stats() == {"tile": 6, "levels": 0}
+        assert a.path_stats() == {"tile": 6, "levels": 0, "graph": 0}
         np.testing.assert_array_equal(a.getH(), b.getH())
         a.set_path(fa.PATH_LEVELS)
         a.set_kernel_timing(True)
@@ -395,7 +397,7 @@ def test_stage_timers_cover_every_stage_on_both_paths():
     with fa.FastScape(W, H, gpu_params(1), engine_seed=0) as m:
         m.reset_timers()
         m.step(20)
-        assert m.path_stats() == {"tile": 20, "levels": 0}
+        assert m.path_stats() == {"tile": 20, "levels": 0, "graph": 0}
         t = m.timers()
         note("tile path timers %s" % t)
         assert all(t[k] > 0 for k in stages), t
@@ -456,7 +458,7 @@ def test_device_pointer_names_the_field_until_the_next_step():
         m.step(1)
         p1 = m.device_h()
         np.testing.assert_array_equal(_hip_dtoh(p1, W * H), m.getH().ravel())
-        assert m.path_stats() == {"tile": 1, "levels": 0} and p1 != p0
+        assert m.path_stats() == {"tile": 1, "levels": 0, "graph": 0} and p1 != p0
         m.step(1)
         p2 = m.device_h()
         assert p2 == p0

@@ -63,7 +63,7 @@ def test_headline_grid_state_synchronised_step(oracle):
     op = oracle.preset(1)
     with fa.FastScape(W, H, bench_params(), engine_seed=0) as m:
         m.step(3)
-        assert m.path_stats() == {"tile": 3, "levels": 0}
+        assert m.path_stats() == {"tile": 3, "levels": 0, "graph": 0}
         h = m.getH().ravel().copy()
         st = oracle.step(h, W, H, op)
         del h
@@ -93,7 +93,7 @@ def test_headline_grid_tile_path_equals_level_path_through_halo_switch():
             np.testing.assert_array_equal(a.levels(), b.levels(), err_msg="levels %d" % upto)
             np.testing.assert_array_equal(a.stack(), b.stack(), err_msg="stack %d" % upto)
         ta = a.tile_stats()
-        assert a.path_stats()["tile"] == 300 and b.path_stats() == {"tile": 0, "levels": 300}
+        assert a.path_stats()["tile"] == 300 and b.path_stats() == {"tile": 0, "levels": 300, "graph": 0}
         assert ta["halo16"] > 200 and ta["halo32"] > 0, ta
 
 

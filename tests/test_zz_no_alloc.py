@@ -9,7 +9,8 @@ exactly the bits of a freshly created model, which proves that what one call
 leaves in the shared array does not leak into the next.  The cases cover the
 tile path with an odd number of steps (the result comes back from the
 second buffer), the 32-cell halo and the level path (the reference preset
-passes 32 levels within a few tens of steps), and n = 2.
+passes 32 levels within a few tens of steps; fs_step_noalloc then takes the
+level path, the model it is compared with the tile-graph path), and n = 2.
 """
 import ctypes
 import sys
@@ -68,7 +69,7 @@ def fs_step_reference(h0, W, H, params, iterations):
         return m.getH().ravel(), m.path_stats(), m.tile_stats()
 
 
-# (W, H, preset, terrain seed, steps, expect level-path steps)
+# (W, H, preset, terrain seed, steps, expect drainage deeper than the halos)
 CASES = [
     (700, 600, fa.PRESET_BENCHMARK, 0, 31, False),
     (300, 200, fa.PRESET_REFERENCE, 4, 40, True),
@@ -86,7 +87,7 @@ def test_noalloc_equals_fs_step_on_consecutive_calls():
             h0 = fa.generate_terrain(W, H, seed)
             want, paths, tiles = fs_step_reference(h0, W, H, p, n)
             if deep:
-                assert paths["levels"] > 0 and tiles["halo32"] > 0, (paths, tiles)
+                assert paths["graph"] > 0 and tiles["halo32"] > 0, (paths, tiles)
             got = noalloc_steps(h0, W, H, p, n, stream if k % 2 else None)
             np.testing.assert_array_equal(got, want, err_msg="call %d %dx%d" % (k, W, H))
         # the small grid once more, right after the large one
