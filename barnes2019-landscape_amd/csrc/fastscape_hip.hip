@@ -148,11 +148,15 @@ struct fs_model {
   uint32_t *node_kids = nullptr, *node_parent = nullptr, *node_out = nullptr;
   int32_t *node_cell = nullptr, *nexit = nullptr;
   uint16_t *node_tier = nullptr;
-  int32_t *tstat = nullptr, *tlist = nullptr;   // per tier: tiles, their list
-  int32_t *tstat_h = nullptr;                   // pinned copy of tstat
+  int32_t *tile_n0 = nullptr;      // per tile: cells with a root tail
+  int32_t *thist = nullptr, *tcur = nullptr;   // per tier: work items, their cursor
+  uint16_t *cells = nullptr;       // per tile: its cells sorted by tier
+  uint64_t *items = nullptr;       // work items in tier order
+  int32_t *tstat = nullptr;        // error bits, deepest tier, items, queue head
+  int graph_blocks = 0;            // persistent erosion grid
   uint64_t graph_steps = 0;
-  int32_t graph_tiers = 0;          // erosion launches of the last graph step
-  int graph_run = 0;                // graph steps since the last tile attempt
+  int graph_run = 0;               // graph steps since the last tile attempt
+  bool graph_in_call = false;      // the current fs_run / fs_step took a graph step
 
   bool ktiming = false;
   std::vector<KTimed> kpending;
@@ -310,9 +314,10 @@ void free_model(fs_model *m) {
                     static_cast<void *>(m->node_kids), static_cast<void *>(m->node_parent),
                     static_cast<void *>(m->node_out), static_cast<void *>(m->node_cell),
                     static_cast<void *>(m->nexit), static_cast<void *>(m->node_tier),
-                    static_cast<void *>(m->tstat), static_cast<void *>(m->tlist)})
+                    static_cast<void *>(m->tstat), static_cast<void *>(m->tile_n0),
+                    static_cast<void *>(m->thist), static_cast<void *>(m->tcur),
+                    static_cast<void *>(m->cells), static_cast<void *>(m->items)})
     if (ptr) (void)hipFree(ptr);
-  if (m->tstat_h) (void)hipHostFree(m->tstat_h);
   if (m->hdr_h) (void)hipHostFree(m->hdr_h);
   if (m->levels_h) (void)hipHostFree(m->levels_h);
   for (auto &set : m->ev)
@@ -785,6 +790,9 @@ int ensure_graph(fs_model *m) {
   const size_t tiles = static_cast<size_t>((m->W + deep::kSide - 1) / deep::kSide) *
                        static_cast<size_t>((m->H + deep::kSide - 1) / deep::kSide);
   const size_t nodes = tiles * deep::kNodeCap;
+  int ncu = 0;
+  FS_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device));
+  m->graph_blocks = 2 * std::max(ncu, 1);
   FS_TRY(fs::dev_alloc(&m->dinfo, N));
   FS_TRY(fs::dev_alloc(&m->link, N));
   FS_TRY(fs::dev_alloc(&m->node_word, nodes));
@@ -794,83 +802,67 @@ int ensure_graph(fs_model *m) {
   FS_TRY(fs::dev_alloc(&m->node_cell, nodes));
   FS_TRY(fs::dev_alloc(&m->node_tier, nodes));
   FS_TRY(fs::dev_alloc(&m->nexit, tiles));
-  FS_TRY(fs::dev_alloc(&m->tstat, deep::kTierWords));
-  FS_TRY(fs::dev_alloc(&m->tlist, static_cast<size_t>(deep::kMaxTier + 1) * tiles));
-  if (hipHostMalloc(reinterpret_cast<void **>(&m->tstat_h), deep::kTierWords * sizeof(int32_t)) !=
-      hipSuccess) {
-    m->tstat_h = nullptr;
-    fs::set_last_error("hipHostMalloc failed");
-    return FS_ERR_OUT_OF_MEMORY;
-  }
+  FS_TRY(fs::dev_alloc(&m->tile_n0, tiles));
+  const size_t tier_words = static_cast<size_t>(deep::kTierCap + 1) * deep::kShards;
+  FS_TRY(fs::dev_alloc(&m->thist, tier_words));
+  FS_TRY(fs::dev_alloc(&m->tcur, tier_words));
+  FS_TRY(fs::dev_alloc(&m->cells, tiles * deep::kCells));
+  FS_TRY(fs::dev_alloc(&m->items, tiles * deep::kTiersPerTile));
+  FS_TRY(fs::dev_alloc(&m->tstat, deep::kStWords));
+  FS_HIP_TRY(hipMemsetAsync(m->thist, 0, tier_words * sizeof(int32_t), m->stream));
   return FS_OK;
 }
 
+// The counts' completion with the erosion of the root-tailed cells, then the
+// persistent erosion of the other tiers.
 template <int NMODE>
 int launch_graph_erosion(fs_model *m, int tiles, int tiles_x) {
   hipStream_t s = m->stream;
-  FS_TRY(kt_begin(m, FS_K_GRAPH_FINAL));
-  hipLaunchKernelGGL(deep::k_deep_final<NMODE>, dim3(tiles), dim3(deep::kBlock), 0, s, m->rcv,
-                     m->cnt, m->dinfo, m->link, m->node_out, m->ftab, m->h, m->h_alt, m->W, m->H,
-                     tiles_x, m->ek, m->hdr);
+  const int W = m->W, H = m->H;
+  FS_TRY(kt_begin(m, FS_K_GRAPH_COUNTS));
+  hipLaunchKernelGGL(deep::k_deep_counts<NMODE>, dim3(tiles), dim3(deep::kBlock), 0, s, m->rcv,
+                     m->cnt, m->dinfo, m->link, m->node_out, m->node_cell, m->node_tier,
+                     m->nexit, m->tile_n0, m->cells, m->tcur, m->items, m->ftab, m->h, m->h_alt,
+                     W, H, tiles_x, m->ek, m->hdr);
   FS_TRY(check_launch());
   FS_TRY(kt_end(m));
-  // the tier counts, read behind the erosion of the root-tailed cells
-  FS_HIP_TRY(hipStreamSynchronize(m->side));
-  const int32_t err = m->tstat_h[0];
-  if (err & deep::kErrTier) return FS_ERR_LEVEL_OVERFLOW;
-  if (err != 0) {
-    fs::set_last_error("tile-graph path: receivers do not form a forest");
-    return FS_ERR_INTERNAL;
-  }
-  const int tmax = m->tstat_h[deep::kMaxTier + 1];
-  for (int t = 1; t <= tmax; t++) {
-    const int n = m->tstat_h[t];
-    if (n <= 0) continue;
-    FS_TRY(kt_begin(m, FS_K_GRAPH_ERODE));
-    hipLaunchKernelGGL(deep::k_deep_erode<NMODE>, dim3(n), dim3(deep::kBlock), 0, s, m->rcv,
-                       m->cnt, m->dinfo, m->node_cell, m->node_tier, m->nexit,
-                       m->tlist + static_cast<size_t>(t) * tiles, m->ftab, m->h, m->h_alt,
-                       m->W, m->H, tiles_x, t, m->ek, m->hdr);
-    FS_TRY(check_launch());
-    FS_TRY(kt_end(m));
-  }
-  m->graph_tiers = tmax;
-  return FS_OK;
+  FS_TRY(kt_begin(m, FS_K_GRAPH_ERODE));
+  hipLaunchKernelGGL(deep::k_deep_erode<NMODE>, dim3(static_cast<unsigned>(m->graph_blocks)),
+                     dim3(deep::kBlock), 0, s, m->rcv, m->cnt, m->dinfo, m->cells, m->items,
+                     m->ftab, m->h, m->h_alt, m->tstat, W, H, tiles_x, m->ek, m->hdr);
+  FS_TRY(check_launch());
+  return kt_end(m);
 }
 
-// The tile-graph path (fs_deep.h): any depth, no halo.  Receivers are
-// already in rcv.  Returns FS_OK with *done = false, having written nothing
-// of the model's state, when the drainage crosses more than kMaxTier tile
-// borders (the caller runs the level path on the same receivers).
-int graph_path(fs_model *m, hipEvent_t *ev, bool *done) {
-  *done = false;
+// The tile-graph path (fs_deep.h): any depth, no halo, no host round trip.
+// Receivers are already in rcv.  Errors (a tier beyond kTierCap, a receiver
+// cycle, a wait that timed out) surface when the call finishes.
+int graph_path(fs_model *m, hipEvent_t *ev) {
   FS_TRY(ensure_graph(m));
   hipStream_t s = m->stream;
-  const int tiles_x = (m->W + deep::kSide - 1) / deep::kSide;
-  const int tiles = tiles_x * ((m->H + deep::kSide - 1) / deep::kSide);
-  hipLaunchKernelGGL(deep::k_deep_clear, dim3(1), dim3(256), 0, s, m->tstat);
+  const int W = m->W, H = m->H;
+  const int tiles_x = (W + deep::kSide - 1) / deep::kSide;
+  const int tiles = tiles_x * ((H + deep::kSide - 1) / deep::kSide);
+  hipLaunchKernelGGL(deep::k_deep_clear, dim3(1), dim3(64), 0, s, m->tstat);
   FS_TRY(check_launch());
   FS_TRY(kt_begin(m, FS_K_GRAPH_LOCAL));
   hipLaunchKernelGGL(deep::k_deep_local, dim3(tiles), dim3(deep::kBlock), 0, s, m->rcv, m->cnt,
                      m->dinfo, m->link, m->node_word, m->node_kids, m->node_cell, m->nexit,
-                     m->tstat, m->W, m->H, tiles_x);
+                     m->tile_n0, m->cells, m->h_alt, m->tstat, W, H, tiles_x);
   FS_TRY(check_launch());
   FS_TRY(kt_end(m));
   FS_TRY(mark(ev[2], s));
   FS_TRY(kt_begin(m, FS_K_GRAPH_NODES));
   hipLaunchKernelGGL(deep::k_deep_links, dim3(tiles), dim3(deep::kNodeCap), 0, s, m->rcv,
-                     m->link, m->node_cell, m->nexit, m->node_parent, m->node_kids, m->W);
+                     m->link, m->node_cell, m->nexit, m->node_parent, m->node_kids, W);
   FS_TRY(check_launch());
   hipLaunchKernelGGL(deep::k_deep_tiers, dim3(tiles), dim3(deep::kNodeCap), 0, s, m->node_parent,
-                     m->nexit, m->node_tier, m->tstat, m->tlist, tiles);
+                     m->nexit, m->node_tier, m->thist, m->tstat);
   FS_TRY(check_launch());
-  FS_TRY(kt_end(m));
-  FS_HIP_TRY(hipEventRecord(m->acc_done, s));
-  FS_HIP_TRY(hipStreamWaitEvent(m->side, m->acc_done, 0));
-  FS_HIP_TRY(hipMemcpyAsync(m->tstat_h, m->tstat, deep::kTierWords * sizeof(int32_t),
-                            hipMemcpyDeviceToHost, m->side));
+  hipLaunchKernelGGL(deep::k_deep_scan, dim3(1), dim3(deep::kBlock), 0, s, m->thist, m->tcur,
+                     m->tstat);
+  FS_TRY(check_launch());
   FS_TRY(mark(ev[3], s));
-  FS_TRY(kt_begin(m, FS_K_GRAPH_NODES));
   hipLaunchKernelGGL(deep::k_deep_sum, dim3(tiles), dim3(deep::kNodeCap), 0, s, m->node_word,
                      m->node_kids, m->node_parent, m->nexit, m->node_out, m->tstat,
                      tiles * deep::kNodeCap);
@@ -878,17 +870,31 @@ int graph_path(fs_model *m, hipEvent_t *ev, bool *done) {
   FS_TRY(kt_end(m));
   FS_TRY(mark(ev[4], s));
   FS_TRY(mark(ev[5], s));
-  int st;
-  if (m->nmode == 1) st = launch_graph_erosion<1>(m, tiles, tiles_x);
-  else if (m->nmode == 2) st = launch_graph_erosion<2>(m, tiles, tiles_x);
-  else st = launch_graph_erosion<0>(m, tiles, tiles_x);
-  if (st == FS_ERR_LEVEL_OVERFLOW) return FS_OK;   // too many tiers: the level path
-  FS_TRY(st);
+  if (m->nmode == 1) FS_TRY(launch_graph_erosion<1>(m, tiles, tiles_x));
+  else if (m->nmode == 2) FS_TRY(launch_graph_erosion<2>(m, tiles, tiles_x));
+  else FS_TRY(launch_graph_erosion<0>(m, tiles, tiles_x));
   std::swap(m->h, m->h_alt);
   m->order_current = false;                 // levels, stack, donor masks on request
   m->cnt_stack = false;
   m->graph_steps++;
-  *done = true;
+  m->graph_in_call = true;
+  return FS_OK;
+}
+
+// Error bits of the graph steps of this call (read once the stream is idle).
+int graph_status(fs_model *m) {
+  if (!m->graph_in_call) return FS_OK;
+  m->graph_in_call = false;
+  int32_t err = 0;
+  FS_HIP_TRY(hipMemcpy(&err, m->tstat + deep::kStErr, sizeof(err), hipMemcpyDeviceToHost));
+  if (err & deep::kErrTier) {
+    fs::set_last_error("tile-graph path: a drainage path crosses more tile borders than levels allow");
+    return FS_ERR_LEVEL_OVERFLOW;
+  }
+  if (err) {
+    fs::set_last_error("tile-graph path: receivers do not form a forest");
+    return FS_ERR_INTERNAL;
+  }
   return FS_OK;
 }
 
@@ -936,8 +942,9 @@ int step_once(fs_model *m) {
   if (m->path == FS_PATH_AUTO && m->tile_halo != 0) FS_TRY(tile_path(m, ev, &done));
   const bool tiled = done;
   if (!done && (m->path == FS_PATH_TILE_GRAPH || (m->path == FS_PATH_AUTO && m->owns_memory))) {
-    FS_TRY(graph_path(m, ev, &done));
-    if (done && m->path == FS_PATH_AUTO) {
+    FS_TRY(graph_path(m, ev));
+    done = true;
+    if (m->path == FS_PATH_AUTO) {
       // drainage this deep rarely shallows again: try the 32-cell halo
       // every kGraphRetry graph steps
       m->tile_halo = ++m->graph_run % kGraphRetry == 0 ? tile32::kHalo : 0;
@@ -962,6 +969,7 @@ int finish(fs_model *m) {
   FS_TRY(stage_collect(m, 0));
   FS_TRY(stage_collect(m, 1));
   FS_TRY(kt_collect(m));
+  FS_TRY(graph_status(m));
   FS_TRY(fetch_header(m, 0));
   return device_error_status(m->hdr_h[2]);
 }
@@ -1278,7 +1286,15 @@ int fs_path_stats(fs_model *m, uint64_t *tile_steps, uint64_t *level_steps) {
 int fs_graph_stats(fs_model *m, uint64_t *graph_steps, int32_t *tiers_last_step) {
   if (!m) return FS_ERR_INVALID_ARGUMENT;
   if (graph_steps) *graph_steps = m->graph_steps;
-  if (tiers_last_step) *tiers_last_step = m->graph_tiers;
+  if (tiers_last_step) {
+    *tiers_last_step = 0;
+    if (m->tstat) {
+      FS_HIP_TRY(hipSetDevice(m->device));
+      FS_HIP_TRY(hipStreamSynchronize(m->stream));
+      FS_HIP_TRY(hipMemcpy(tiers_last_step, m->tstat + deep::kStMaxTier, sizeof(int32_t),
+                           hipMemcpyDeviceToHost));
+    }
+  }
   return FS_OK;
 }
 
@@ -1294,7 +1310,7 @@ const char *fs_kernel_name(int kernel) {
                                               "k_bfs",       "k_flowacc",    "k_uplift_roots",
                                               "k_erode",     "k_jump",       "k_tile_acc",
                                               "k_tile_erode", "k_deep_local", "k_deep_nodes",
-                                              "k_deep_final", "k_deep_erode"};
+                                              "k_deep_counts", "k_deep_erode"};
   if (kernel < 0 || kernel >= FS_NUM_KERNELS) return "unknown";
   return names[kernel];
 }

@@ -7,31 +7,45 @@
 // the tile (a root) or the last tile cell before the chain leaves the tile
 // (an exit).  Exits are the nodes of a forest that spans the tiles: the
 // parent of exit x is the tail of x's receiver when that tail is an exit.
+// A node's tier is 1 + its parent's (1 when x's receiver has a root tail);
+// a cell's tier is its tail's (0 for a root tail).  A tier-t cell can be
+// eroded once the tier t-1 cells are: its chain ends, inside its tile, at
+// an exit whose receiver is a tier t-1 cell of another tile.
 //
 //   k_deep_local  per tile: the in-tile walks give every cell its local count
 //                 (the tile cells upstream of it along in-tile paths, itself
 //                 included), its tail and its hops to the tail; exits become
-//                 nodes; border cells record the node of their tail (link)
+//                 nodes; border cells record the node of their tail (link);
+//                 each exit's receiver gets the "not yet eroded" mark in the
+//                 new elevation field; the cells whose tail is a root start
+//                 the tile's cell list, sorted by hops
 //   k_deep_links  every node's parent, and the number of children per node
-//   k_deep_tiers  every node's tier: 1 + its parent's, 1 under a root tail.
-//                 Cells whose tail is a tier-t exit are eroded by launch t;
-//                 per tier, the list of tiles that hold such cells
+//   k_deep_tiers  every node's tier (walking its parents), and the number of
+//                 (tile, tier) work items per tier
+//   k_deep_scan   the work items' offsets by tier (one workgroup)
 //   k_deep_sum    the count through every exit: its local count plus its
 //                 children's, leaf to root (the last child to arrive carries
 //                 the sum on, so each node is summed once)
-//   k_deep_final  per tile: the counts finished by the inflows of the ring
-//                 cells that drain into the tile, each walked down its
-//                 in-tile chain; then the erosion of the cells with a root
-//                 tail, level by level from the root
-//   k_deep_erode  launch t >= 1, per listed tile: the erosion of the cells
-//                 whose tail is a tier-t exit, from the exit's receiver
-//                 (eroded by launch t-1) up
+//   k_deep_counts per tile: the counts finished by the inflows of the ring
+//                 cells that drain into the tile (each walked down its
+//                 in-tile chain); the erosion of the cells with a root tail
+//                 (tier 0) in LDS, from the roots up; the tile's other cells
+//                 grouped by tier into the cell list, one work item per tier
+//                 placed in tier order
+//   k_deep_erode  a persistent grid takes the work items (tiers 1 and up) in
+//                 tier order and erodes each item's cells from its exits up:
+//                 an exit waits until its receiver's new elevation replaces
+//                 the mark; a wave's cells wait on their receivers' done
+//                 flags in LDS
 //
 // Counts are integers, so the order of the additions is free: they equal
 // the reference's ComputeFlowAcc (RB:262-279) counts.  Every cell is eroded
 // after its receiver with the reference's Newton iteration (RB:306-328) from
 // the same inputs (its elevation plus uplift, the receiver's new elevation,
 // the factor of its count), so the new elevations are the reference's bits.
+// Work items wait only on items of the tier below, which the queue handed
+// out earlier to running workgroups, so the persistent grid always drains;
+// every wait is bounded all the same and reported in hdr[2].
 #ifndef FS_DEEP_H
 #define FS_DEEP_H
 
@@ -42,6 +56,7 @@ constexpr int kCells = kSide * kSide;
 constexpr int kBlock = 1024;
 constexpr int kPer = kCells / kBlock;
 constexpr int kNodeCap = 256;             // nodes per tile: a border has at most 252 cells
+constexpr int kTiersPerTile = kNodeCap;   // distinct tiers in one tile: its nodes' and 0
 constexpr uint16_t kTermNone = 0xFFFD;    // beyond the grid
 constexpr uint16_t kTermExit = 0xFFFE;    // the receiver lies in another tile
 constexpr uint16_t kTermRoot = 0xFFFF;    // NO_FLOW
@@ -49,15 +64,31 @@ constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 constexpr int kSumShift = 40;             // node word: arrivals << 40 | count
 constexpr uint64_t kSumMask = (uint64_t{1} << kSumShift) - 1;
 constexpr uint64_t kOneArrival = uint64_t{1} << kSumShift;
-constexpr int kMaxTier = 1023;            // erosion launches per step after the first
-constexpr int kTierWords = kMaxTier + 2;  // [0] error bits, [1..kMaxTier] tiles, [kMaxTier+1] max
+constexpr int kTierCap = FS_MAX_LEVELS;   // a tier is at most the depth of the drainage
+// tstat words
+constexpr int kStErr = 0, kStMaxTier = 1, kStItems = 2, kStQueue = 3, kStWords = 4;
+// per-tier item counters are split over kShards words (by workgroup id, so
+// by XCD) to keep tens of thousands of atomics per step off one address
+constexpr int kShards = 8;
+constexpr int kBatch = 4;                 // work items a workgroup takes at once
 constexpr int32_t kErrTier = 1, kErrChain = 2;
+// the "not yet eroded" mark: a signalling NaN that no arithmetic produces
+constexpr unsigned long long kUnsolved = 0x7FF4A5A5A5A5A5A5ull;
+constexpr int kSpinCap = 1 << 24;
 static_assert(kCells <= 4096, "tail and hops fit 12 bits each");
+static_assert(kTierCap < 65536, "tiers fit 16 bits");
 
 // dinfo word of a cell: tail (12 bits) | hops to the tail << 12 | exit tail
 constexpr uint32_t kInfoExit = 1u << 24;
 __device__ __forceinline__ int info_tail(uint32_t w) { return static_cast<int>(w & 0xFFFu); }
 __device__ __forceinline__ int info_hops(uint32_t w) { return static_cast<int>((w >> 12) & 0xFFFu); }
+
+// work item: tile (27 bits) | first cell in the tile's sorted list << 27 |
+// number of cells << 40
+__device__ __forceinline__ uint64_t item_pack(int t, int first, int n) {
+  return static_cast<uint64_t>(t) | (static_cast<uint64_t>(first) << 27) |
+         (static_cast<uint64_t>(n) << 40);
+}
 
 // Receiver direction r = 0..7 (W, NW, N, NE, E, SE, S, SW) as a column and
 // row step.
@@ -102,6 +133,34 @@ __device__ __forceinline__ int block_max(int v, int *slot) {
   return *slot;
 }
 
+// Exclusive scan of v over the workgroup (kBlock threads); *total gets the
+// sum.  sh holds kBlock / kWave + 1 ints.  Ends with a barrier.
+__device__ __forceinline__ int block_scan(int v, int *sh, int *total) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  int x = v;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == kWave - 1) sh[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int w = 0; w < kBlock / kWave; w++) {
+      const int c = sh[w];
+      sh[w] = run;
+      run += c;
+    }
+    sh[kBlock / kWave] = run;
+  }
+  __syncthreads();
+  const int r = sh[wave] + x - v;
+  *total = sh[kBlock / kWave];
+  __syncthreads();
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // k_deep_local: one tile per workgroup
 // ---------------------------------------------------------------------------
@@ -109,14 +168,16 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
     const uint8_t *__restrict__ rcv, uint32_t *__restrict__ cnt, uint32_t *__restrict__ dinfo,
     uint32_t *__restrict__ link, uint64_t *__restrict__ node_word,
     uint32_t *__restrict__ node_kids, int32_t *__restrict__ node_cell,
-    int32_t *__restrict__ nexit, int32_t *__restrict__ tstat, int W, int H, int tiles_x) {
+    int32_t *__restrict__ nexit, int32_t *__restrict__ tile_n0, uint16_t *__restrict__ cells,
+    double *__restrict__ h_new, int32_t *__restrict__ tstat, int W, int H, int tiles_x) {
   __shared__ uint16_t nx[kCells];
+  __shared__ uint32_t jw[kCells];    // pointer jumping: ancestor | hops to it << 16
   __shared__ uint32_t lc[kCells];
   __shared__ uint16_t nid[kCells];   // node number of each exit within the tile
-  __shared__ int nnode;
+  __shared__ int nnode, maxhops, scan[kBlock / kWave + 1];
   const int t = blockIdx.x;
   const Box b = tile_box(t, tiles_x, W, H);
-  if (threadIdx.x == 0) nnode = 0;
+  if (threadIdx.x == 0) nnode = maxhops = 0;
 #pragma unroll
   for (int j = 0; j < kPer; j++) {
     const int i = j * kBlock + threadIdx.x;
@@ -125,40 +186,54 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
     if (lx < b.tw && ly < b.th)
       n = tile_next(rcv[static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx], lx, ly, b);
     nx[i] = n;
-    lc[i] = 0;
+    jw[i] = n < kTermNone ? (static_cast<uint32_t>(n) | (1u << 16)) : static_cast<uint32_t>(i);
+    lc[i] = n != kTermNone ? 1u : 0u;
   }
   __syncthreads();
-  // every cell walks its in-tile chain and adds 1 to each cell it passes,
-  // itself and its tail included; the thread's walkers advance together
-  int cur[kPer], hops[kPer];
-  uint32_t live = 0;
-#pragma unroll
-  for (int j = 0; j < kPer; j++) {
-    const int i = j * kBlock + threadIdx.x;
-    cur[j] = i;
-    hops[j] = 0;
-    if (nx[i] != kTermNone) live |= 1u << j;
-  }
-  bool bad = false;
-  while (live) {
+  // tail and hops of every cell by pointer jumping: each round doubles the
+  // hops a pointer covers (reading a word another thread rewrote in the
+  // same round is still a valid ancestor and distance)
+  bool bad = true;
+  for (int round = 0; round <= 12; round++) {
+    bool moved = false;
 #pragma unroll
     for (int j = 0; j < kPer; j++) {
-      if (!((live >> j) & 1u)) continue;
-      atomicAdd(&lc[cur[j]], 1u);
-      const uint16_t n = nx[cur[j]];
-      if (n >= kTermNone) {
-        live &= ~(1u << j);
-      } else if (hops[j] + 1 >= kCells) {   // a receiver cycle: not a forest
-        bad = true;
-        live &= ~(1u << j);
-      } else {
-        cur[j] = n;
-        hops[j]++;
-      }
+      const int i = j * kBlock + threadIdx.x;
+      const uint32_t w = jw[i];
+      const int a = static_cast<int>(w & 0xFFFFu);
+      if (a == i) continue;
+      const uint32_t wa = jw[a];
+      if (static_cast<int>(wa & 0xFFFFu) == a) continue;
+      jw[i] = (wa & 0xFFFFu) | ((w >> 16) + (wa >> 16)) << 16;
+      moved = true;
+    }
+    if (!__syncthreads_or(moved)) {
+      bad = false;
+      break;
     }
   }
-  if (bad) atomicOr(&tstat[0], kErrChain);
-  __syncthreads();
+  if (bad && threadIdx.x == 0) atomicOr(&tstat[kStErr], kErrChain);   // a receiver cycle
+  int cur[kPer], hops[kPer];
+  int mh = 0;
+#pragma unroll
+  for (int j = 0; j < kPer; j++) {
+    const uint32_t w = jw[j * kBlock + threadIdx.x];
+    cur[j] = static_cast<int>(w & 0xFFFu);
+    hops[j] = static_cast<int>(w >> 16) & 0xFFF;
+    mh = max(mh, hops[j]);
+  }
+  const int top = block_max(mh, &maxhops);
+  // local counts from the top hops down: a cell's in-tile donors are one hop
+  // further from the tail, so their counts are complete when it adds its own
+  for (int L = top; L >= 1; L--) {
+#pragma unroll
+    for (int j = 0; j < kPer; j++)
+      if (hops[j] == L) {
+        const int i = j * kBlock + threadIdx.x;
+        atomicAdd(&lc[nx[i]], lc[i]);
+      }
+    __syncthreads();
+  }
   // exits become the tile's nodes
 #pragma unroll
   for (int j = 0; j < kPer; j++) {
@@ -173,9 +248,12 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
       const int k = base + __popcll(m & ((uint64_t{1} << lane) - 1));
       nid[i] = static_cast<uint16_t>(k);
       const int id = t * kNodeCap + k;
-      node_cell[id] = (b.y0 + i / kSide) * W + b.x0 + i % kSide;
+      const int32_t g = (b.y0 + i / kSide) * W + b.x0 + i % kSide;
+      node_cell[id] = g;
       node_word[id] = lc[i];
       node_kids[id] = 0;
+      const uint8_t r = rcv[g];
+      h_new[g + dir_offset(r, W)] = __longlong_as_double(static_cast<long long>(kUnsolved));
     }
   }
   __syncthreads();
@@ -193,6 +271,34 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
       link[g] = exit_tail ? static_cast<uint32_t>(t * kNodeCap + nid[cur[j]]) : kNoNode;
   }
   if (threadIdx.x == 0) nexit[t] = nnode;
+  // the cells with a root tail, sorted by hops, start the tile's cell list
+  // (jw is free now: its words count the cells per hops)
+  for (int q = threadIdx.x; q <= top; q += kBlock) jw[q] = 0;
+  __syncthreads();
+  bool rooted[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; j++) {
+    const int i = j * kBlock + threadIdx.x;
+    rooted[j] = nx[i] != kTermNone && nx[cur[j]] == kTermRoot;
+    if (rooted[j]) atomicAdd(&jw[hops[j]], 1u);
+  }
+  __syncthreads();
+  int carry = 0;
+  for (int base = 0; base <= top; base += kBlock) {
+    const int q = base + threadIdx.x;
+    const int v = q <= top ? static_cast<int>(jw[q]) : 0;
+    int total;
+    const int ex = block_scan(v, scan, &total);
+    if (q <= top) jw[q] = static_cast<uint32_t>(carry + ex);
+    carry += total;
+  }
+  __syncthreads();   // every cursor is set before any thread takes a slot
+  uint16_t *const list = cells + static_cast<size_t>(t) * kCells;
+#pragma unroll
+  for (int j = 0; j < kPer; j++)
+    if (rooted[j])
+      list[atomicAdd(&jw[hops[j]], 1u)] = static_cast<uint16_t>(j * kBlock + threadIdx.x);
+  if (threadIdx.x == 0) tile_n0[t] = carry;
 }
 
 // ---------------------------------------------------------------------------
@@ -213,12 +319,11 @@ __global__ __launch_bounds__(kNodeCap) void k_deep_links(
 }
 
 // Tier of every node, by walking its parents (a chain of tiles crossed by
-// one drainage path); tiles listed once per tier they hold.
+// one drainage path); per tier, the number of tiles holding cells of it.
 __global__ __launch_bounds__(kNodeCap) void k_deep_tiers(
     const uint32_t *__restrict__ node_parent, const int32_t *__restrict__ nexit,
-    uint16_t *__restrict__ node_tier, int32_t *__restrict__ tstat, int32_t *__restrict__ tlist,
-    int ntiles) {
-  __shared__ int tier_s[kNodeCap];
+    uint16_t *__restrict__ node_tier, int32_t *__restrict__ thist, int32_t *__restrict__ tstat) {
+  __shared__ uint32_t seen[(kTierCap + 32) / 32];   // tiers already counted for the tile
   const int t = blockIdx.x, k = threadIdx.x;
   const int n = nexit[t];
   int tier = 0;
@@ -226,24 +331,52 @@ __global__ __launch_bounds__(kNodeCap) void k_deep_tiers(
     const int id = t * kNodeCap + k;
     tier = 1;
     uint32_t p = node_parent[id];
-    while (p != kNoNode && tier <= kMaxTier) {
+    while (p != kNoNode && tier <= kTierCap) {
       tier++;
       p = node_parent[p];
     }
-    node_tier[id] = static_cast<uint16_t>(min(tier, kMaxTier + 1));
-    tier_s[k] = tier;
+    node_tier[id] = static_cast<uint16_t>(min(tier, kTierCap));
   }
+  // clear the words the tile's tiers can touch (the block's own entries)
+  if (k < n && tier <= kTierCap) seen[tier >> 5] = 0;
   __syncthreads();
   if (k >= n) return;
-  if (tier > kMaxTier) {
-    atomicOr(&tstat[0], kErrTier);
+  if (tier > kTierCap) {
+    atomicOr(&tstat[kStErr], kErrTier);
     return;
   }
-  for (int q = 0; q < k; q++)
-    if (tier_s[q] == tier) return;   // an earlier node of the tile lists it
-  const int pos = atomicAdd(&tstat[tier], 1);
-  tlist[static_cast<size_t>(tier) * ntiles + pos] = t;
-  atomicMax(&tstat[kMaxTier + 1], tier);
+  const uint32_t bit = 1u << (tier & 31);
+  if (atomicOr(&seen[tier >> 5], bit) & bit) return;   // another node of the tile counts it
+  atomicAdd(&thist[tier * kShards + (t & (kShards - 1))], 1);
+  if (tier > __hip_atomic_load(&tstat[kStMaxTier], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(&tstat[kStMaxTier], tier);
+}
+
+// Offsets of the work items by tier: thist (counts per tier and shard) ->
+// tcur (first slot of each, then the cursor k_deep_counts appends at), so
+// the items of a tier are contiguous and the tiers in order; clears thist
+// for the next step and the queue head.  One workgroup of kBlock threads.
+__global__ __launch_bounds__(kBlock) void k_deep_scan(int32_t *__restrict__ thist,
+                                                      int32_t *__restrict__ tcur,
+                                                      int32_t *__restrict__ tstat) {
+  __shared__ int sh[kBlock / kWave + 1];
+  const int n = (tstat[kStMaxTier] + 1) * kShards;   // tier-major, shard-minor
+  int carry = 0;
+  for (int base = 0; base < n; base += kBlock) {
+    const int k = base + threadIdx.x;
+    const int v = k < n ? thist[k] : 0;
+    int total;
+    const int ex = block_scan(v, sh, &total);
+    if (k < n) {
+      tcur[k] = carry + ex;
+      thist[k] = 0;
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) {
+    tstat[kStItems] = carry;
+    tstat[kStQueue] = 0;
+  }
 }
 
 // Count through every exit, leaf to root.
@@ -265,50 +398,88 @@ __global__ __launch_bounds__(kNodeCap) void k_deep_sum(
     val += old & kSumMask;
     id = p;
   }
-  atomicOr(&tstat[0], kErrChain);
+  atomicOr(&tstat[kStErr], kErrChain);
 }
 
 // ---------------------------------------------------------------------------
-// erosion
+// k_deep_counts: one tile per workgroup
 // ---------------------------------------------------------------------------
-struct FinalShared {
-  double hs[kCells];
+struct CountShared {
+  double hs[kCells];              // new elevations of the tier-0 cells
   uint32_t cs[kCells];
   uint16_t nx[kCells];
-  int maxhops;
+  uint16_t tier_at[kCells];       // tier of each exit cell
+  uint8_t done[kCells];
+  int tiers[kTiersPerTile];       // the tile's distinct tiers, then sorted
+  int sorted[kTiersPerTile];
+  int hist[kTiersPerTile];        // cells per tier rank, then their cursor
+  int scan[kBlock / kWave + 1];
+  int ntier;
 };
 
+__device__ __forceinline__ int tier_rank(const int *sorted, int n, int tier) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (sorted[mid] < tier) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
 template <int NMODE>
-__global__ __launch_bounds__(kBlock) void k_deep_final(
+__global__ __launch_bounds__(kBlock) void k_deep_counts(
     const uint8_t *__restrict__ rcv, uint32_t *__restrict__ cnt,
     const uint32_t *__restrict__ dinfo, const uint32_t *__restrict__ link,
-    const uint32_t *__restrict__ node_out, const double *__restrict__ ftab,
+    const uint32_t *__restrict__ node_out, const int32_t *__restrict__ node_cell,
+    const uint16_t *__restrict__ node_tier, const int32_t *__restrict__ nexit,
+    const int32_t *__restrict__ tile_n0, uint16_t *__restrict__ cells,
+    int32_t *__restrict__ tcur, uint64_t *__restrict__ items, const double *__restrict__ ftab,
     const double *__restrict__ h_old, double *__restrict__ h_new, int W, int H, int tiles_x,
     ErodeConst k, int32_t *hdr) {
-  __shared__ FinalShared sh;
+  __shared__ CountShared sh;
   const int t = blockIdx.x;
   const Box b = tile_box(t, tiles_x, W, H);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   uint32_t info[kPer];
-  uint8_t rr[kPer];
-  if (threadIdx.x == 0) sh.maxhops = 0;
+  if (threadIdx.x == 0) sh.ntier = 0;
+  if (threadIdx.x < kTiersPerTile) sh.hist[threadIdx.x] = 0;
 #pragma unroll
   for (int j = 0; j < kPer; j++) {
     const int i = j * kBlock + threadIdx.x;
     const int lx = i % kSide, ly = i / kSide;
     uint16_t n = kTermNone;
-    rr[j] = kNoFlow;
     info[j] = 0;
     if (lx < b.tw && ly < b.th) {
       const size_t g = static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx;
-      rr[j] = rcv[g];
-      n = tile_next(rr[j], lx, ly, b);
+      n = tile_next(rcv[g], lx, ly, b);
       info[j] = dinfo[g];
       sh.cs[i] = cnt[g];
       sh.hs[i] = h_old[g];
     }
     sh.nx[i] = n;
+    sh.done[i] = 0;
+  }
+  const int n = nexit[t];
+  int my_tier = -1;
+  if (static_cast<int>(threadIdx.x) < n) {
+    const int id = t * kNodeCap + threadIdx.x;
+    const int32_t g = node_cell[id];
+    my_tier = node_tier[id];
+    sh.tier_at[(g / W - b.y0) * kSide + (g % W - b.x0)] = static_cast<uint16_t>(my_tier);
+    sh.tiers[threadIdx.x] = my_tier;
   }
   __syncthreads();
+  // the tile's distinct exit tiers, sorted
+  if (my_tier >= 0) {
+    bool first = true;
+    for (int q = 0; q < static_cast<int>(threadIdx.x); q++)
+      if (sh.tiers[q] == my_tier) {
+        first = false;
+        break;
+      }
+    if (first) sh.sorted[atomicAdd(&sh.ntier, 1)] = my_tier;
+  }
   // inflows: a ring cell that drains into the tile is an exit of its own
   // tile; its count is added along the in-tile chain from the cell it
   // drains into
@@ -341,47 +512,98 @@ __global__ __launch_bounds__(kBlock) void k_deep_final(
             int cur = ty * kSide + tx;
             for (int hop = 0; hop < kCells; hop++) {
               atomicAdd(&sh.cs[cur], v);
-              const uint16_t n = sh.nx[cur];
-              if (n >= kTermNone) break;
-              cur = n;
+              const uint16_t nn = sh.nx[cur];
+              if (nn >= kTermNone) break;
+              cur = nn;
             }
           }
         }
       }
     }
   }
-  int mh = 0;
-#pragma unroll
-  for (int j = 0; j < kPer; j++)
-    if (sh.nx[j * kBlock + threadIdx.x] != kTermNone && !(info[j] & kInfoExit))
-      mh = max(mh, info_hops(info[j]));
-  const int maxhops = block_max(mh, &sh.maxhops);   // ends with a barrier
+  __syncthreads();
+  const int ntier = sh.ntier;
+  if (static_cast<int>(threadIdx.x) < ntier) {
+    const int v = sh.sorted[threadIdx.x];
+    int rank = 0;
+    for (int q = 0; q < ntier; q++) rank += sh.sorted[q] < v;
+    sh.tiers[rank] = v;   // tiers[] no longer needed as node tiers
+  }
+  __syncthreads();
+  // final counts; each exit-tailed cell's tier rank
+  int rank[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; j++) {
     const int i = j * kBlock + threadIdx.x;
+    rank[j] = -1;
     if (sh.nx[i] == kTermNone) continue;
     cnt[static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide] = sh.cs[i];
+    if (!(info[j] & kInfoExit)) continue;
+    rank[j] = tier_rank(sh.tiers, ntier, sh.tier_at[info_tail(info[j])]);
+    atomicAdd(&sh.hist[rank[j]], 1);
   }
-  // cells with a root tail, level by level from the root (hops 0): the root
-  // keeps its elevation, plus uplift off rings 0 and 1
-  for (int L = 0; L <= maxhops; L++) {
-#pragma unroll
-    for (int j = 0; j < kPer; j++) {
-      const int i = j * kBlock + threadIdx.x;
-      if (sh.nx[i] == kTermNone || (info[j] & kInfoExit) || info_hops(info[j]) != L) continue;
-      const int lx = i % kSide, ly = i / kSide;
-      if (L == 0) {
-        const int gx = b.x0 + lx, gy = b.y0 + ly;
-        if (gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2) sh.hs[i] += k.ud;
-      } else {
-        const int r = rr[j];
-        const int nb = i + dir_dy(r) * kSide + dir_dx(r);
-        const double fact = ftab[2 * static_cast<size_t>(sh.cs[i]) + (r & 1)];
-        sh.hs[i] = newton_update<NMODE>(sh.hs[i] + k.ud, sh.hs[nb], fact, k, hdr);
-      }
+  __syncthreads();
+  const int n0 = tile_n0[t];
+  {
+    const int v = static_cast<int>(threadIdx.x) < ntier ? sh.hist[threadIdx.x] : 0;
+    int total;
+    const int ex = block_scan(v, sh.scan, &total);
+    if (static_cast<int>(threadIdx.x) < ntier) {
+      sh.hist[threadIdx.x] = n0 + ex;   // cursor
+      // the work item of this tier
+      const int tier = sh.tiers[threadIdx.x];
+      const int pos = atomicAdd(&tcur[tier * kShards + (t & (kShards - 1))], 1);
+      items[pos] = item_pack(t, n0 + ex, v);
     }
-    __syncthreads();
   }
+  __syncthreads();   // every cursor is set before any thread takes a slot
+  uint16_t *const list = cells + static_cast<size_t>(t) * kCells;
+#pragma unroll
+  for (int j = 0; j < kPer; j++) {
+    if (rank[j] < 0) continue;
+    const int pos = atomicAdd(&sh.hist[rank[j]], 1);
+    list[pos] = static_cast<uint16_t>(j * kBlock + threadIdx.x);
+  }
+  // tier 0: the cells with a root tail, from the list k_deep_local sorted
+  // by hops; each wave takes 64-cell chunks in order and a lane solves its
+  // cell once its receiver (an earlier cell of the list) is done
+  for (int ch = wave; ch * kWave < n0; ch += kBlock / kWave) {
+    const int p = ch * kWave + lane;
+    if (p >= n0) continue;
+    const int i = list[p];
+    const int ri = sh.nx[i];
+    const bool root = ri == kTermRoot;
+    double fact = 0.0;
+    if (!root) {
+      const int d = ri - i;   // a diagonal step is +-63 or +-65 cells away
+      const bool diag = d != 1 && d != -1 && d != kSide && d != -kSide;
+      fact = ftab[2 * static_cast<size_t>(sh.cs[i]) + (diag ? 1 : 0)];
+    }
+    const double h0 = sh.hs[i];
+    int spins = 0;
+    for (;;) {
+      const bool ready = root || __hip_atomic_load(&sh.done[ri], __ATOMIC_ACQUIRE,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+      const bool give_up = !ready && ++spins > kSpinCap;
+      if (ready || give_up) {
+        double hn = h0;
+        if (give_up) {
+          atomicOr(&hdr[2], kErrLookback);
+        } else if (root) {
+          const int gx = b.x0 + i % kSide, gy = b.y0 + i / kSide;
+          if (gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2) hn += k.ud;
+        } else {
+          hn = newton_update<NMODE>(h0 + k.ud, sh.hs[ri], fact, k, hdr);
+        }
+        sh.hs[i] = hn;
+        __hip_atomic_store(&sh.done[i], static_cast<uint8_t>(1), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
 #pragma unroll
   for (int j = 0; j < kPer; j++) {
     const int i = j * kBlock + threadIdx.x;
@@ -390,83 +612,155 @@ __global__ __launch_bounds__(kBlock) void k_deep_final(
   }
 }
 
+// ---------------------------------------------------------------------------
+// k_deep_erode: persistent; one work item at a time per workgroup
+// ---------------------------------------------------------------------------
 struct ErodeShared {
-  double hs[kCells];
-  uint16_t tier_at[kCells];   // tier of each exit cell
+  double hs[kCells];        // new elevation of each tile cell of the item
+  uint16_t list[kCells];    // the item's cells sorted by hops
+  int hist[kCells];         // cells per hops, then cursors
+  uint8_t done[kCells];
+  int scan[kBlock / kWave + 1];
+  uint64_t item;
   int maxhops;
 };
 
-// Launch `tier` >= 1: the listed tiles' cells whose tail is a tier-`tier`
-// exit.  h_new holds every cell of lower tiers (earlier launches).
 template <int NMODE>
 __global__ __launch_bounds__(kBlock) void k_deep_erode(
     const uint8_t *__restrict__ rcv, const uint32_t *__restrict__ cnt,
-    const uint32_t *__restrict__ dinfo, const int32_t *__restrict__ node_cell,
-    const uint16_t *__restrict__ node_tier, const int32_t *__restrict__ nexit,
-    const int32_t *__restrict__ list, const double *__restrict__ ftab,
-    const double *__restrict__ h_old, double *h_new, int W, int H, int tiles_x, int tier,
+    const uint32_t *__restrict__ dinfo, const uint16_t *__restrict__ cells,
+    const uint64_t *__restrict__ items, const double *__restrict__ ftab,
+    const double *__restrict__ h_old, double *h_new, int32_t *tstat, int W, int H, int tiles_x,
     ErodeConst k, int32_t *hdr) {
   __shared__ ErodeShared sh;
-  const int t = list[blockIdx.x];
-  const Box b = tile_box(t, tiles_x, W, H);
-  uint32_t info[kPer];
-  uint8_t rr[kPer];
-  bool valid[kPer];
-  if (threadIdx.x == 0) sh.maxhops = 0;
-#pragma unroll
-  for (int j = 0; j < kPer; j++) {
-    const int i = j * kBlock + threadIdx.x;
-    const int lx = i % kSide, ly = i / kSide;
-    valid[j] = lx < b.tw && ly < b.th;
-    info[j] = 0;
-    rr[j] = kNoFlow;
-    if (valid[j]) {
-      const size_t g = static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx;
-      info[j] = dinfo[g];
-      rr[j] = rcv[g];
-      sh.hs[i] = h_old[g];
-    }
-  }
-  const int n = nexit[t];
-  if (static_cast<int>(threadIdx.x) < n) {
-    const int id = t * kNodeCap + threadIdx.x;
-    const int32_t g = node_cell[id];
-    const int i = (g / W - b.y0) * kSide + (g % W - b.x0);
-    sh.tier_at[i] = node_tier[id];
-  }
-  __syncthreads();
-  bool mine[kPer];
-  int mh = 0;
-#pragma unroll
-  for (int j = 0; j < kPer; j++) {
-    mine[j] = valid[j] && (info[j] & kInfoExit) && sh.tier_at[info_tail(info[j])] == tier;
-    if (mine[j]) mh = max(mh, info_hops(info[j]));
-  }
-  const int maxhops = block_max(mh, &sh.maxhops);
-  for (int L = 0; L <= maxhops; L++) {
-#pragma unroll
-    for (int j = 0; j < kPer; j++) {
-      if (!mine[j] || info_hops(info[j]) != L) continue;
-      const int i = j * kBlock + threadIdx.x;
-      const int r = rr[j];
-      const size_t g = static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide;
-      const double hr = L == 0 ? h_new[static_cast<long long>(g) + dir_offset(r, W)]
-                               : sh.hs[i + dir_dy(r) * kSide + dir_dx(r)];
-      const double fact = ftab[2 * static_cast<size_t>(cnt[g]) + (r & 1)];
-      sh.hs[i] = newton_update<NMODE>(sh.hs[i] + k.ud, hr, fact, k, hdr);
+  const int nitems = tstat[kStItems];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  int next = 0, last = 0;   // this workgroup's batch of items [next, last)
+  for (;;) {
+    if (threadIdx.x == 0) {
+      if (next == last) {
+        next = atomicAdd(&tstat[kStQueue], kBatch);
+        last = min(next + kBatch, nitems);
+      }
+      sh.item = next < last ? items[next] : ~uint64_t{0};
+      next++;
+      sh.maxhops = 0;
     }
     __syncthreads();
-  }
+    const uint64_t item = sh.item;
+    if (item == ~uint64_t{0}) return;
+    const int t = static_cast<int>(item & ((1u << 27) - 1));
+    const int first = static_cast<int>((item >> 27) & 0x1FFF);
+    const int n = static_cast<int>(item >> 40);
+    const Box b = tile_box(t, tiles_x, W, H);
+    const uint16_t *const tl = cells + static_cast<size_t>(t) * kCells + first;
+    // the item's cells and their hops
+    int loc[kPer], hp[kPer];
+    int mh = 0;
 #pragma unroll
-  for (int j = 0; j < kPer; j++) {
-    if (!mine[j]) continue;
-    const int i = j * kBlock + threadIdx.x;
-    h_new[static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide] = sh.hs[i];
+    for (int j = 0; j < kPer; j++) {
+      const int s = j * kBlock + threadIdx.x;
+      sh.hist[s] = 0;
+      loc[j] = -1;
+      hp[j] = 0;
+      if (s < n) {
+        loc[j] = tl[s];
+        hp[j] = info_hops(dinfo[static_cast<size_t>(b.y0 + loc[j] / kSide) * W + b.x0 +
+                                loc[j] % kSide]);
+        mh = max(mh, hp[j]);
+      }
+    }
+    const int maxhops = block_max(mh, &sh.maxhops);
+#pragma unroll
+    for (int j = 0; j < kPer; j++)
+      if (loc[j] >= 0) {
+        atomicAdd(&sh.hist[hp[j]], 1);
+        sh.done[loc[j]] = 0;
+      }
+    __syncthreads();
+    // hops offsets, kBlock at a time
+    {
+      int carry = 0;
+      for (int base = 0; base <= maxhops; base += kBlock) {
+        const int q = base + threadIdx.x;
+        const int v = q <= maxhops ? sh.hist[q] : 0;
+        int total;
+        const int ex = block_scan(v, sh.scan, &total);
+        if (q <= maxhops) sh.hist[q] = carry + ex;
+        carry += total;
+      }
+    }
+    __syncthreads();   // every cursor is set before any thread takes a slot
+#pragma unroll
+    for (int j = 0; j < kPer; j++)
+      if (loc[j] >= 0) sh.list[atomicAdd(&sh.hist[hp[j]], 1)] = static_cast<uint16_t>(loc[j]);
+    __syncthreads();
+    // wavefront: each wave takes 64-cell chunks in order; a lane solves its
+    // cell as soon as its receiver is done (an earlier cell of the list, or
+    // for an exit a cell of a lower tier in another tile).  The wait, the
+    // solve and the publication share one loop, so a lane whose receiver is
+    // on its own wave gets it on a later pass.
+    for (int ch = wave; ch * kWave < n; ch += kBlock / kWave) {
+      const int p = ch * kWave + lane;
+      if (p >= n) continue;
+      const int i = sh.list[p];
+      const int lx = i % kSide, ly = i / kSide;
+      const size_t g = static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx;
+      const uint8_t r = rcv[g];
+      const double h0 = h_old[g];
+      const bool root = r >= 8;
+      const bool ext = !root && info_hops(dinfo[g]) == 0;   // receiver in another tile
+      const double fact = root ? 0.0 : ftab[2 * static_cast<size_t>(cnt[g]) + (r & 1)];
+      const int ri = root ? i : i + dir_dy(r) * kSide + dir_dx(r);
+      const unsigned long long *rp = reinterpret_cast<const unsigned long long *>(
+          h_new + static_cast<long long>(g) + (root ? 0 : dir_offset(r, W)));
+      int spins = 0;
+      for (;;) {
+        bool ready = root;
+        double hr = 0.0;
+        if (ext) {
+          const unsigned long long v =
+              __hip_atomic_load(rp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ready = v != kUnsolved;
+          hr = __longlong_as_double(static_cast<long long>(v));
+        } else if (!root) {
+          ready = __hip_atomic_load(&sh.done[ri], __ATOMIC_ACQUIRE,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+          if (ready) hr = sh.hs[ri];
+        }
+        const bool give_up = !ready && ++spins > kSpinCap;
+        if (ready || give_up) {
+          double hn = h0;
+          if (give_up) {
+            atomicOr(&hdr[2], kErrLookback);
+          } else if (root) {
+            if (b.x0 + lx >= 2 && b.x0 + lx < W - 2 && b.y0 + ly >= 2 && b.y0 + ly < H - 2)
+              hn += k.ud;
+          } else {
+            hn = newton_update<NMODE>(h0 + k.ud, hr, fact, k, hdr);
+          }
+          sh.hs[i] = hn;
+          __hip_atomic_store(&sh.done[i], static_cast<uint8_t>(1), __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+          // every new elevation goes past this XCD's cache: border cells may
+          // be the receivers of other tiles' exits, and a cache line holds
+          // cells of two tiles wherever a row does not start on a line, so
+          // a plain store beside such a receiver would leave this XCD's L2
+          // with a copy of the line that its polls could read
+          __hip_atomic_store(reinterpret_cast<unsigned long long *>(h_new + g),
+                             static_cast<unsigned long long>(__double_as_longlong(hn)),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __syncthreads();
   }
 }
 
 __global__ void k_deep_clear(int32_t *tstat) {
-  for (int i = threadIdx.x; i < kTierWords; i += blockDim.x) tstat[i] = 0;
+  if (threadIdx.x < kStWords) tstat[threadIdx.x] = 0;
 }
 
 }  // namespace deep

@@ -23,7 +23,7 @@ TIMER_NAMES = ["Step1: Initialize", "Step2: DetermineReceivers", "Step3: Determi
                "Overall"]
 KERNELS = ["k_receivers", "k_topology", "k_roots", "k_bfs", "k_flowacc", "k_uplift_roots",
            "k_erode", "k_jump", "k_tile_acc", "k_tile_erode", "k_deep_local", "k_deep_nodes",
-           "k_deep_final", "k_deep_erode"]
+           "k_deep_counts", "k_deep_erode"]
 PATH_AUTO = 0
 PATH_LEVELS = 1
 PATH_TILE_GRAPH = 2
@@ -371,8 +371,8 @@ class FastScape:
         return {"tile": t.value, "levels": lv.value, "graph": self.graph_stats()["steps"]}
 
     def graph_stats(self):
-        """Tile-graph steps and the erosion launches after the first that the
-        last one needed: {"steps": n, "tiers_last_step": k}."""
+        """Tile-graph steps and the most tile borders one drainage path
+        crossed in the last of them: {"steps": n, "tiers_last_step": k}."""
         n = ctypes.c_uint64(0)
         k = ctypes.c_int32(0)
         _check(_lib().fs_graph_stats(self._m, ctypes.byref(n), ctypes.byref(k)), "fs_graph_stats")

@@ -161,10 +161,10 @@ int fs_reset_timers(fs_model *m);
  * halo; otherwise (or when a tile's chains need more halo cells than its
  * plan holds: 256 with the 16-cell halo, 512 with the 32-cell one) the step
  * takes the tile-graph path: per 64x64 tile with no halo, the tiles joined
- * by the graph of the cells where drainage leaves a tile, one erosion
- * launch per tile border that the longest drainage path crosses (at most
- * 1023, beyond which the step takes the level path).  Deep drainage stays
- * on the tile-graph path and retries the 32-cell halo every 64 steps.
+ * by the graph of the cells where drainage leaves a tile, the erosion one
+ * persistent launch that takes (tile, tier) work items in the order of the
+ * tile borders crossed on the way to an outlet.  Deep drainage stays on the
+ * tile-graph path and retries the 32-cell halo every 64 steps.
  * Models over lent buffers (fs_step_noalloc) take the level path instead.
  * FS_PATH_LEVELS: always the level path -- one launch per level for order,
  * accumulation and erosion, as the reference's stages (RB:366-372).
@@ -182,8 +182,8 @@ int fs_path_stats(fs_model *m, uint64_t *tile_steps, uint64_t *level_steps);
 /* Tile-path steps split by halo (16-cell, 32-cell).  No reference
  * counterpart. */
 int fs_tile_stats(fs_model *m, uint64_t *halo16_steps, uint64_t *halo32_steps);
-/* Tile-graph path steps, and the erosion launches after the first that its
- * last step needed (the most tile borders one drainage path crossed).  No
+/* Tile-graph path steps, and the most tile borders one drainage path
+ * crossed in the last of them (synchronises the model's stream).  No
  * reference counterpart. */
 int fs_graph_stats(fs_model *m, uint64_t *graph_steps, int32_t *tiers_last_step);
 
@@ -250,7 +250,7 @@ int fs_synchronize(fs_model *m);
 #define FS_K_TILE_ERODE 9
 #define FS_K_GRAPH_LOCAL 10
 #define FS_K_GRAPH_NODES 11
-#define FS_K_GRAPH_FINAL 12
+#define FS_K_GRAPH_COUNTS 12
 #define FS_K_GRAPH_ERODE 13
 #define FS_NUM_KERNELS 14
 int fs_set_kernel_timing(fs_model *m, int enable);
