@@ -792,7 +792,7 @@ int ensure_graph(fs_model *m) {
   const size_t nodes = tiles * deep::kNodeCap;
   int ncu = 0;
   FS_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device));
-  m->graph_blocks = 2 * std::max(ncu, 1);
+  m->graph_blocks = 4 * std::max(ncu, 1);   // four k_deep_erode workgroups per CU
   FS_TRY(fs::dev_alloc(&m->dinfo, N));
   FS_TRY(fs::dev_alloc(&m->link, N));
   FS_TRY(fs::dev_alloc(&m->node_word, nodes));
@@ -828,7 +828,7 @@ int launch_graph_erosion(fs_model *m, int tiles, int tiles_x) {
   FS_TRY(kt_end(m));
   FS_TRY(kt_begin(m, FS_K_GRAPH_ERODE));
   hipLaunchKernelGGL(deep::k_deep_erode<NMODE>, dim3(static_cast<unsigned>(m->graph_blocks)),
-                     dim3(deep::kBlock), 0, s, m->rcv, m->cnt, m->dinfo, m->cells, m->items,
+                     dim3(deep::kEroBlock), 0, s, m->rcv, m->cnt, m->dinfo, m->cells, m->items,
                      m->ftab, m->h, m->h_alt, m->tstat, W, H, tiles_x, m->ek, m->hdr);
   FS_TRY(check_launch());
   return kt_end(m);

@@ -414,7 +414,7 @@ struct CountShared {
   int sorted[kTiersPerTile];
   int hist[kTiersPerTile];        // cells per tier rank, then their cursor
   int scan[kBlock / kWave + 1];
-  int ntier;
+  int ntier, maxhops;
 };
 
 __device__ __forceinline__ int tier_rank(const int *sorted, int n, int tier) {
@@ -428,7 +428,7 @@ __device__ __forceinline__ int tier_rank(const int *sorted, int n, int tier) {
 }
 
 template <int NMODE>
-__global__ __launch_bounds__(kBlock) void k_deep_counts(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_deep_counts(
     const uint8_t *__restrict__ rcv, uint32_t *__restrict__ cnt,
     const uint32_t *__restrict__ dinfo, const uint32_t *__restrict__ link,
     const uint32_t *__restrict__ node_out, const int32_t *__restrict__ node_cell,
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void k_deep_counts(
   const Box b = tile_box(t, tiles_x, W, H);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   uint32_t info[kPer];
-  if (threadIdx.x == 0) sh.ntier = 0;
+  if (threadIdx.x == 0) sh.ntier = sh.maxhops = 0;
   if (threadIdx.x < kTiersPerTile) sh.hist[threadIdx.x] = 0;
 #pragma unroll
   for (int j = 0; j < kPer; j++) {
@@ -455,7 +455,6 @@ __global__ __launch_bounds__(kBlock) void k_deep_counts(
       n = tile_next(rcv[g], lx, ly, b);
       info[j] = dinfo[g];
       sh.cs[i] = cnt[g];
-      sh.hs[i] = h_old[g];
     }
     sh.nx[i] = n;
     sh.done[i] = 0;
@@ -544,26 +543,68 @@ __global__ __launch_bounds__(kBlock) void k_deep_counts(
   }
   __syncthreads();
   const int n0 = tile_n0[t];
+  int mh = 0;
+#pragma unroll
+  for (int j = 0; j < kPer; j++)
+    if (rank[j] >= 0) mh = max(mh, info_hops(info[j]));
+  const int span = block_max(mh, &sh.maxhops) + 1;   // hops values per tier
   {
+    // the work items: tier ranks in order, each its cells' count
     const int v = static_cast<int>(threadIdx.x) < ntier ? sh.hist[threadIdx.x] : 0;
     int total;
     const int ex = block_scan(v, sh.scan, &total);
     if (static_cast<int>(threadIdx.x) < ntier) {
-      sh.hist[threadIdx.x] = n0 + ex;   // cursor
-      // the work item of this tier
       const int tier = sh.tiers[threadIdx.x];
       const int pos = atomicAdd(&tcur[tier * kShards + (t & (kShards - 1))], 1);
       items[pos] = item_pack(t, n0 + ex, v);
+      sh.hist[threadIdx.x] = n0 + ex;   // first slot of the rank
     }
   }
-  __syncthreads();   // every cursor is set before any thread takes a slot
+  // each rank's cells sorted by hops (a work item solves its cells in list
+  // order): one counting sort over (rank, hops) keys when they fit the
+  // elevation array, which is not loaded yet; else one per rank
+  int *const keyc = reinterpret_cast<int *>(sh.hs);
+  constexpr int kKeys = static_cast<int>(sizeof(sh.hs) / sizeof(int));
   uint16_t *const list = cells + static_cast<size_t>(t) * kCells;
+  const int passes = ntier * span <= kKeys ? 1 : ntier;
+  for (int pass = 0; pass < passes; pass++) {
+    const int nkeys = passes == 1 ? ntier * span : span;
+    for (int q = threadIdx.x; q < nkeys; q += kBlock) keyc[q] = 0;
+    __syncthreads();
+    int key[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      key[j] = -1;
+      if (rank[j] < 0 || (passes > 1 && rank[j] != pass)) continue;
+      key[j] = (passes == 1 ? rank[j] * span : 0) + info_hops(info[j]);
+      atomicAdd(&keyc[key[j]], 1);
+    }
+    __syncthreads();
+    int carry = passes == 1 ? 0 : sh.hist[pass];
+    if (passes == 1) carry = n0;
+    for (int base = 0; base < nkeys; base += kBlock) {
+      const int q = base + threadIdx.x;
+      const int v = q < nkeys ? keyc[q] : 0;
+      int total;
+      const int ex = block_scan(v, sh.scan, &total);
+      if (q < nkeys) keyc[q] = carry + ex;
+      carry += total;
+    }
+    __syncthreads();   // every cursor is set before any thread takes a slot
+#pragma unroll
+    for (int j = 0; j < kPer; j++)
+      if (key[j] >= 0)
+        list[atomicAdd(&keyc[key[j]], 1)] = static_cast<uint16_t>(j * kBlock + threadIdx.x);
+    __syncthreads();
+  }
+  // the tile's elevations, now that the sort's scratch is free
 #pragma unroll
   for (int j = 0; j < kPer; j++) {
-    if (rank[j] < 0) continue;
-    const int pos = atomicAdd(&sh.hist[rank[j]], 1);
-    list[pos] = static_cast<uint16_t>(j * kBlock + threadIdx.x);
+    const int i = j * kBlock + threadIdx.x;
+    if (sh.nx[i] != kTermNone)
+      sh.hs[i] = h_old[static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide];
   }
+  __syncthreads();
   // tier 0: the cells with a root tail, from the list k_deep_local sorted
   // by hops; each wave takes 64-cell chunks in order and a lane solves its
   // cell once its receiver (an earlier cell of the list) is done
@@ -615,18 +656,17 @@ __global__ __launch_bounds__(kBlock) void k_deep_counts(
 // ---------------------------------------------------------------------------
 // k_deep_erode: persistent; one work item at a time per workgroup
 // ---------------------------------------------------------------------------
+constexpr int kEroBlock = 256;   // four workgroups per CU: items are small and latency-bound
 struct ErodeShared {
   double hs[kCells];        // new elevation of each tile cell of the item
-  uint16_t list[kCells];    // the item's cells sorted by hops
-  int hist[kCells];         // cells per hops, then cursors
   uint8_t done[kCells];
-  int scan[kBlock / kWave + 1];
   uint64_t item;
-  int maxhops;
 };
 
+// Work items of tiers 1 and up; an item's cells come sorted by hops, so a
+// cell's receiver in the tile (one hop nearer the exit) is always earlier.
 template <int NMODE>
-__global__ __launch_bounds__(kBlock) void k_deep_erode(
+__global__ __launch_bounds__(kEroBlock) void k_deep_erode(
     const uint8_t *__restrict__ rcv, const uint32_t *__restrict__ cnt,
     const uint32_t *__restrict__ dinfo, const uint16_t *__restrict__ cells,
     const uint64_t *__restrict__ items, const double *__restrict__ ftab,
@@ -644,7 +684,6 @@ __global__ __launch_bounds__(kBlock) void k_deep_erode(
       }
       sh.item = next < last ? items[next] : ~uint64_t{0};
       next++;
-      sh.maxhops = 0;
     }
     __syncthreads();
     const uint64_t item = sh.item;
@@ -654,76 +693,36 @@ __global__ __launch_bounds__(kBlock) void k_deep_erode(
     const int n = static_cast<int>(item >> 40);
     const Box b = tile_box(t, tiles_x, W, H);
     const uint16_t *const tl = cells + static_cast<size_t>(t) * kCells + first;
-    // the item's cells and their hops
-    int loc[kPer], hp[kPer];
-    int mh = 0;
-#pragma unroll
-    for (int j = 0; j < kPer; j++) {
-      const int s = j * kBlock + threadIdx.x;
-      sh.hist[s] = 0;
-      loc[j] = -1;
-      hp[j] = 0;
-      if (s < n) {
-        loc[j] = tl[s];
-        hp[j] = info_hops(dinfo[static_cast<size_t>(b.y0 + loc[j] / kSide) * W + b.x0 +
-                                loc[j] % kSide]);
-        mh = max(mh, hp[j]);
-      }
-    }
-    const int maxhops = block_max(mh, &sh.maxhops);
-#pragma unroll
-    for (int j = 0; j < kPer; j++)
-      if (loc[j] >= 0) {
-        atomicAdd(&sh.hist[hp[j]], 1);
-        sh.done[loc[j]] = 0;
-      }
-    __syncthreads();
-    // hops offsets, kBlock at a time
-    {
-      int carry = 0;
-      for (int base = 0; base <= maxhops; base += kBlock) {
-        const int q = base + threadIdx.x;
-        const int v = q <= maxhops ? sh.hist[q] : 0;
-        int total;
-        const int ex = block_scan(v, sh.scan, &total);
-        if (q <= maxhops) sh.hist[q] = carry + ex;
-        carry += total;
-      }
-    }
-    __syncthreads();   // every cursor is set before any thread takes a slot
-#pragma unroll
-    for (int j = 0; j < kPer; j++)
-      if (loc[j] >= 0) sh.list[atomicAdd(&sh.hist[hp[j]], 1)] = static_cast<uint16_t>(loc[j]);
+    for (int p = threadIdx.x; p < n; p += kEroBlock) sh.done[tl[p]] = 0;
     __syncthreads();
     // wavefront: each wave takes 64-cell chunks in order; a lane solves its
     // cell as soon as its receiver is done (an earlier cell of the list, or
     // for an exit a cell of a lower tier in another tile).  The wait, the
     // solve and the publication share one loop, so a lane whose receiver is
     // on its own wave gets it on a later pass.
-    for (int ch = wave; ch * kWave < n; ch += kBlock / kWave) {
+    for (int ch = wave; ch * kWave < n; ch += kEroBlock / kWave) {
       const int p = ch * kWave + lane;
       if (p >= n) continue;
-      const int i = sh.list[p];
+      const int i = tl[p];
       const int lx = i % kSide, ly = i / kSide;
       const size_t g = static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx;
       const uint8_t r = rcv[g];
       const double h0 = h_old[g];
-      const bool root = r >= 8;
-      const bool ext = !root && info_hops(dinfo[g]) == 0;   // receiver in another tile
-      const double fact = root ? 0.0 : ftab[2 * static_cast<size_t>(cnt[g]) + (r & 1)];
-      const int ri = root ? i : i + dir_dy(r) * kSide + dir_dx(r);
+      const bool ext = info_hops(dinfo[g]) == 0;   // an exit: receiver in another tile
+      const double fact = ftab[2 * static_cast<size_t>(cnt[g]) + (r & 1)];
+      const int ri = i + dir_dy(r) * kSide + dir_dx(r);
       const unsigned long long *rp = reinterpret_cast<const unsigned long long *>(
-          h_new + static_cast<long long>(g) + (root ? 0 : dir_offset(r, W)));
+          h_new + static_cast<long long>(g) + dir_offset(r, W));
       int spins = 0;
       for (;;) {
-        bool ready = root;
+        bool ready;
         double hr = 0.0;
         if (ext) {
           const unsigned long long v =
               __hip_atomic_load(rp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           ready = v != kUnsolved;
           hr = __longlong_as_double(static_cast<long long>(v));
-        } else if (!root) {
+        } else {
           ready = __hip_atomic_load(&sh.done[ri], __ATOMIC_ACQUIRE,
                                     __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
           if (ready) hr = sh.hs[ri];
@@ -731,22 +730,16 @@ __global__ __launch_bounds__(kBlock) void k_deep_erode(
         const bool give_up = !ready && ++spins > kSpinCap;
         if (ready || give_up) {
           double hn = h0;
-          if (give_up) {
-            atomicOr(&hdr[2], kErrLookback);
-          } else if (root) {
-            if (b.x0 + lx >= 2 && b.x0 + lx < W - 2 && b.y0 + ly >= 2 && b.y0 + ly < H - 2)
-              hn += k.ud;
-          } else {
-            hn = newton_update<NMODE>(h0 + k.ud, hr, fact, k, hdr);
-          }
+          if (give_up) atomicOr(&hdr[2], kErrLookback);
+          else hn = newton_update<NMODE>(h0 + k.ud, hr, fact, k, hdr);
           sh.hs[i] = hn;
           __hip_atomic_store(&sh.done[i], static_cast<uint8_t>(1), __ATOMIC_RELEASE,
                              __HIP_MEMORY_SCOPE_WORKGROUP);
-          // every new elevation goes past this XCD's cache: border cells may
-          // be the receivers of other tiles' exits, and a cache line holds
-          // cells of two tiles wherever a row does not start on a line, so
-          // a plain store beside such a receiver would leave this XCD's L2
-          // with a copy of the line that its polls could read
+          // every new elevation goes past this XCD's cache: border cells
+          // may be the receivers of other tiles' exits, and a cache line
+          // holds cells of two tiles wherever a row does not start on a
+          // line, so a plain store beside such a receiver would leave this
+          // XCD's L2 with a copy of the line that its polls could read
           __hip_atomic_store(reinterpret_cast<unsigned long long *>(h_new + g),
                              static_cast<unsigned long long>(__double_as_longlong(hn)),
                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
