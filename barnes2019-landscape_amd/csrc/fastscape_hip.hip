@@ -154,6 +154,8 @@ struct fs_model {
   uint64_t *items = nullptr;       // work items in tier order
   int32_t *tstat = nullptr;        // error bits, deepest tier, items, queue head
   int graph_blocks = 0;            // persistent erosion grid
+  deep::GraphGrid *graph_grid = nullptr;   // this model as the tile-graph kernels' one grid
+  int32_t *graph_list = nullptr;           // its work list: first[2], which[1], size[2]
   uint64_t graph_steps = 0;
   int graph_run = 0;               // graph steps since the last tile attempt
   bool graph_in_call = false;      // the current fs_run / fs_step took a graph step
@@ -316,7 +318,8 @@ void free_model(fs_model *m) {
                     static_cast<void *>(m->nexit), static_cast<void *>(m->node_tier),
                     static_cast<void *>(m->tstat), static_cast<void *>(m->tile_n0),
                     static_cast<void *>(m->thist), static_cast<void *>(m->tcur),
-                    static_cast<void *>(m->cells), static_cast<void *>(m->items)})
+                    static_cast<void *>(m->cells), static_cast<void *>(m->items),
+                    static_cast<void *>(m->graph_grid), static_cast<void *>(m->graph_list)})
     if (ptr) (void)hipFree(ptr);
   if (m->hdr_h) (void)hipHostFree(m->hdr_h);
   if (m->levels_h) (void)hipHostFree(m->levels_h);
@@ -810,26 +813,51 @@ int ensure_graph(fs_model *m) {
   FS_TRY(fs::dev_alloc(&m->items, tiles * deep::kTiersPerTile));
   FS_TRY(fs::dev_alloc(&m->tstat, deep::kStWords));
   FS_HIP_TRY(hipMemsetAsync(m->thist, 0, tier_words * sizeof(int32_t), m->stream));
+  deep::GraphGrid g{};
+  g.off = 0;
+  g.ftab_off = 0;
+  g.hdr = m->hdr;
+  g.levels = nullptr;
+  g.W = m->W;
+  g.H = m->H;
+  g.tiles_x = (m->W + deep::kSide - 1) / deep::kSide;
+  g.tile0 = 0;
+  g.nlev = 0;
+  g.nmode = m->nmode;
+  g.ek = m->ek;
+  const int32_t list[5] = {0, static_cast<int32_t>(tiles), 0, 1, static_cast<int32_t>(tiles)};
+  FS_TRY(fs::dev_alloc(&m->graph_grid, 1));
+  FS_TRY(fs::dev_alloc(&m->graph_list, 5));
+  FS_HIP_TRY(hipMemcpy(m->graph_grid, &g, sizeof(g), hipMemcpyHostToDevice));
+  FS_HIP_TRY(hipMemcpy(m->graph_list, list, sizeof(list), hipMemcpyHostToDevice));
   return FS_OK;
+}
+
+deep::GraphWork graph_work(fs_model *m) {
+  deep::GraphWork w;
+  w.grid = m->graph_grid;
+  w.first = m->graph_list;        // {0, tiles}
+  w.which = m->graph_list + 2;    // {0}
+  w.size = m->graph_list + 3;     // {1 entry, tiles}
+  return w;
 }
 
 // The counts' completion with the erosion of the root-tailed cells, then the
 // persistent erosion of the other tiers.
 template <int NMODE>
-int launch_graph_erosion(fs_model *m, int tiles, int tiles_x) {
+int launch_graph_erosion(fs_model *m, int tiles) {
   hipStream_t s = m->stream;
-  const int W = m->W, H = m->H;
+  const deep::GraphWork w = graph_work(m);
   FS_TRY(kt_begin(m, FS_K_GRAPH_COUNTS));
-  hipLaunchKernelGGL(deep::k_deep_counts<NMODE>, dim3(tiles), dim3(deep::kBlock), 0, s, m->rcv,
+  hipLaunchKernelGGL(deep::k_deep_counts<NMODE>, dim3(tiles), dim3(deep::kBlock), 0, s, w, m->rcv,
                      m->cnt, m->dinfo, m->link, m->node_out, m->node_cell, m->node_tier,
-                     m->nexit, m->tile_n0, m->cells, m->tcur, m->items, m->ftab, m->h, m->h_alt,
-                     W, H, tiles_x, m->ek, m->hdr);
+                     m->nexit, m->tile_n0, m->cells, m->tcur, m->items, m->ftab, m->h, m->h_alt);
   FS_TRY(check_launch());
   FS_TRY(kt_end(m));
   FS_TRY(kt_begin(m, FS_K_GRAPH_ERODE));
   hipLaunchKernelGGL(deep::k_deep_erode<NMODE>, dim3(static_cast<unsigned>(m->graph_blocks)),
-                     dim3(deep::kEroBlock), 0, s, m->rcv, m->cnt, m->dinfo, m->cells, m->items,
-                     m->ftab, m->h, m->h_alt, m->tstat, W, H, tiles_x, m->ek, m->hdr);
+                     dim3(deep::kEroBlock), 0, s, w, m->rcv, m->cnt, m->dinfo, m->cells, m->items,
+                     m->ftab, m->h, m->h_alt, m->tstat);
   FS_TRY(check_launch());
   return kt_end(m);
 }
@@ -840,39 +868,40 @@ int launch_graph_erosion(fs_model *m, int tiles, int tiles_x) {
 int graph_path(fs_model *m, hipEvent_t *ev) {
   FS_TRY(ensure_graph(m));
   hipStream_t s = m->stream;
-  const int W = m->W, H = m->H;
-  const int tiles_x = (W + deep::kSide - 1) / deep::kSide;
-  const int tiles = tiles_x * ((H + deep::kSide - 1) / deep::kSide);
+  const int tiles = ((m->W + deep::kSide - 1) / deep::kSide) *
+                    ((m->H + deep::kSide - 1) / deep::kSide);
+  const deep::GraphWork w = graph_work(m);
   hipLaunchKernelGGL(deep::k_deep_clear, dim3(1), dim3(64), 0, s, m->tstat);
   FS_TRY(check_launch());
   FS_TRY(kt_begin(m, FS_K_GRAPH_LOCAL));
-  hipLaunchKernelGGL(deep::k_deep_local, dim3(tiles), dim3(deep::kBlock), 0, s, m->rcv, m->cnt,
+  hipLaunchKernelGGL(deep::k_deep_local, dim3(tiles), dim3(deep::kBlock), 0, s, w, m->rcv, m->cnt,
                      m->dinfo, m->link, m->node_word, m->node_kids, m->node_cell, m->nexit,
-                     m->tile_n0, m->cells, m->h_alt, m->tstat, W, H, tiles_x);
+                     m->tile_n0, m->cells, m->h_alt, m->tstat);
   FS_TRY(check_launch());
   FS_TRY(kt_end(m));
   FS_TRY(mark(ev[2], s));
   FS_TRY(kt_begin(m, FS_K_GRAPH_NODES));
-  hipLaunchKernelGGL(deep::k_deep_links, dim3(tiles), dim3(deep::kNodeCap), 0, s, m->rcv,
-                     m->link, m->node_cell, m->nexit, m->node_parent, m->node_kids, W);
+  hipLaunchKernelGGL(deep::k_deep_links, dim3(tiles), dim3(deep::kNodeCap), 0, s, w, m->rcv,
+                     m->link, m->dinfo, m->node_cell, m->nexit, m->node_parent, m->node_kids,
+                     static_cast<uint32_t *>(nullptr));
   FS_TRY(check_launch());
-  hipLaunchKernelGGL(deep::k_deep_tiers, dim3(tiles), dim3(deep::kNodeCap), 0, s, m->node_parent,
-                     m->nexit, m->node_tier, m->thist, m->tstat);
+  hipLaunchKernelGGL(deep::k_deep_tiers, dim3(tiles), dim3(deep::kNodeCap), 0, s, w,
+                     m->node_parent, m->nexit, m->node_tier, static_cast<const uint32_t *>(nullptr),
+                     static_cast<uint32_t *>(nullptr), m->thist, m->tstat);
   FS_TRY(check_launch());
   hipLaunchKernelGGL(deep::k_deep_scan, dim3(1), dim3(deep::kBlock), 0, s, m->thist, m->tcur,
                      m->tstat);
   FS_TRY(check_launch());
   FS_TRY(mark(ev[3], s));
-  hipLaunchKernelGGL(deep::k_deep_sum, dim3(tiles), dim3(deep::kNodeCap), 0, s, m->node_word,
-                     m->node_kids, m->node_parent, m->nexit, m->node_out, m->tstat,
-                     tiles * deep::kNodeCap);
+  hipLaunchKernelGGL(deep::k_deep_sum, dim3(tiles), dim3(deep::kNodeCap), 0, s, w, m->node_word,
+                     m->node_kids, m->node_parent, m->nexit, m->node_out, m->tstat);
   FS_TRY(check_launch());
   FS_TRY(kt_end(m));
   FS_TRY(mark(ev[4], s));
   FS_TRY(mark(ev[5], s));
-  if (m->nmode == 1) FS_TRY(launch_graph_erosion<1>(m, tiles, tiles_x));
-  else if (m->nmode == 2) FS_TRY(launch_graph_erosion<2>(m, tiles, tiles_x));
-  else FS_TRY(launch_graph_erosion<0>(m, tiles, tiles_x));
+  if (m->nmode == 1) FS_TRY(launch_graph_erosion<1>(m, tiles));
+  else if (m->nmode == 2) FS_TRY(launch_graph_erosion<2>(m, tiles));
+  else FS_TRY(launch_graph_erosion<0>(m, tiles));
   std::swap(m->h, m->h_alt);
   m->order_current = false;                 // levels, stack, donor masks on request
   m->cnt_stack = false;

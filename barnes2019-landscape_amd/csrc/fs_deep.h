@@ -1,6 +1,7 @@
 // fs_deep.h -- the tile-graph path: a time-step for drainage of any depth,
-// tile by tile with no halo.  Included once by fastscape_hip.hip, after
-// fs_device.h.
+// tile by tile with no halo, for one grid or a list of grids.  Included by
+// fastscape_hip.hip and fs_ensemble.hip after fs_device.h; everything here
+// has internal linkage.
 //
 // Within a 64x64 tile, each cell's receiver chain is followed only while it
 // stays in the tile.  The chain ends at the cell's tail: a NO_FLOW cell of
@@ -49,6 +50,7 @@
 #ifndef FS_DEEP_H
 #define FS_DEEP_H
 
+namespace {
 namespace deep {
 
 constexpr int kSide = 64;
@@ -125,6 +127,89 @@ __device__ __forceinline__ uint16_t tile_next(uint8_t r, int lx, int ly, const B
   return static_cast<uint16_t>(ny * kSide + nx);
 }
 
+// A grid the kernels step: its cells from `off` in the per-cell arrays, its
+// tiles from `tile0` in the per-tile arrays (node ids are global: tile *
+// kNodeCap + k), its erosion factors from `ftab_off`.
+struct GraphGrid {
+  uint64_t off, ftab_off;
+  int32_t *hdr;       // hdr[2]: error bits; hdr[0]: deepest level (when levels are kept)
+  int32_t *levels;    // cells per level (nlev entries), or nullptr
+  int W, H, tiles_x, tile0, nlev, nmode;
+  ErodeConst ek;
+};
+// The grids of one step: the tiles of the listed grids, numbered from 0 in
+// list order (the work index).
+struct GraphWork {
+  const GraphGrid *grid;
+  const int32_t *first;    // per entry: its first work index; entry size[0] holds the total
+  const int32_t *which;    // per entry: its grid
+  const int32_t *size;     // [0] entries, [1] tiles
+};
+// The values a workgroup needs of its tile's grid, held in scalar registers
+// (the descriptor is read once per tile; readfirstlane tells the compiler
+// the values are the same in every lane).
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+  return (static_cast<uint64_t>(static_cast<uint32_t>(uni(static_cast<int>(v >> 32)))) << 32) |
+         static_cast<uint32_t>(uni(static_cast<int>(v)));
+}
+template <class T>
+__device__ __forceinline__ T *uniptr(T *p) {
+  return reinterpret_cast<T *>(uni64(reinterpret_cast<uint64_t>(p)));
+}
+__device__ __forceinline__ double unid(double v) {
+  return __longlong_as_double(static_cast<long long>(uni64(static_cast<uint64_t>(__double_as_longlong(v)))));
+}
+struct WorkTile {
+  uint64_t off, ftab_off;
+  int32_t *hdr, *levels;
+  int W, H, tiles_x, nlev, nmode;
+  int t;    // tile within its grid
+  int tg;   // tile in the per-tile arrays
+  ErodeConst ek;
+};
+__device__ __forceinline__ WorkTile work_tile(const GraphWork &w, int wi) {
+  int lo = 0, hi = uni(w.size[0]) - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (uni(w.first[mid]) <= wi) lo = mid;
+    else hi = mid - 1;
+  }
+  const GraphGrid *const g = w.grid + uni(w.which[lo]);
+  WorkTile r;
+  r.t = wi - uni(w.first[lo]);
+  r.tg = uni(g->tile0) + r.t;
+  r.off = uni64(g->off);
+  r.ftab_off = uni64(g->ftab_off);
+  r.hdr = uniptr(g->hdr);
+  r.levels = uniptr(g->levels);
+  r.W = uni(g->W);
+  r.H = uni(g->H);
+  r.tiles_x = uni(g->tiles_x);
+  r.nlev = uni(g->nlev);
+  r.nmode = uni(g->nmode);
+  r.ek.kdt = unid(g->ek.kdt);
+  r.ek.lenpow0 = unid(g->ek.lenpow0);
+  r.ek.lenpow1 = unid(g->ek.lenpow1);
+  r.ek.ud = unid(g->ek.ud);
+  r.ek.neq = unid(g->ek.neq);
+  r.ek.tol = unid(g->ek.tol);
+  return r;
+}
+
+// The Newton solve in the grid's pow mode (NMODE < 0: chosen per grid).
+template <int NMODE>
+__device__ __forceinline__ double solve(int nm, double h0, double hn, double fact,
+                                        const ErodeConst &k, int32_t *hdr) {
+  if constexpr (NMODE >= 0) {
+    return newton_update<NMODE>(h0, hn, fact, k, hdr);
+  } else {
+    if (nm == 1) return newton_update<1>(h0, hn, fact, k, hdr);
+    if (nm == 2) return newton_update<2>(h0, hn, fact, k, hdr);
+    return newton_update<0>(h0, hn, fact, k, hdr);
+  }
+}
+
 __device__ __forceinline__ int block_max(int v, int *slot) {
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) v = max(v, __shfl_xor(v, d));
@@ -165,19 +250,31 @@ __device__ __forceinline__ int block_scan(int v, int *sh, int *total) {
 // k_deep_local: one tile per workgroup
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_deep_local(
-    const uint8_t *__restrict__ rcv, uint32_t *__restrict__ cnt, uint32_t *__restrict__ dinfo,
-    uint32_t *__restrict__ link, uint64_t *__restrict__ node_word,
+    GraphWork w, const uint8_t *__restrict__ rcv0, uint32_t *__restrict__ cnt0,
+    uint32_t *__restrict__ dinfo0, uint32_t *__restrict__ link0, uint64_t *__restrict__ node_word,
     uint32_t *__restrict__ node_kids, int32_t *__restrict__ node_cell,
     int32_t *__restrict__ nexit, int32_t *__restrict__ tile_n0, uint16_t *__restrict__ cells,
-    double *__restrict__ h_new, int32_t *__restrict__ tstat, int W, int H, int tiles_x) {
+    double *__restrict__ h_new0, int32_t *__restrict__ tstat) {
   __shared__ uint16_t nx[kCells];
   __shared__ uint32_t jw[kCells];    // pointer jumping: ancestor | hops to it << 16
   __shared__ uint32_t lc[kCells];
   __shared__ uint16_t nid[kCells];   // node number of each exit within the tile
   __shared__ int nnode, maxhops, scan[kBlock / kWave + 1];
-  const int t = blockIdx.x;
-  const Box b = tile_box(t, tiles_x, W, H);
+  const int wi = blockIdx.x;   // one tile per workgroup; the grid covers the list exactly
+  if (wi >= w.size[1]) return;
+  const WorkTile wt = work_tile(w, wi);
+  const WorkTile &G = wt;
+  const int W = G.W, H = G.H;
+  const uint8_t *const rcv = rcv0 + G.off;
+  uint32_t *const cnt = cnt0 + G.off, *const dinfo = dinfo0 + G.off, *const link = link0 + G.off;
+  double *const h_new = h_new0 + G.off;
+  const int t = wt.tg;   // per-tile arrays and node ids
+  const Box b = tile_box(wt.t, G.tiles_x, W, H);
   if (threadIdx.x == 0) nnode = maxhops = 0;
+  if (wt.t == 0 && G.levels) {   // the grid's level counts start from zero
+    if (threadIdx.x == 0) G.hdr[0] = 0;
+    for (int q = threadIdx.x; q < G.nlev; q += kBlock) G.levels[q] = 0;
+  }
 #pragma unroll
   for (int j = 0; j < kPer; j++) {
     const int i = j * kBlock + threadIdx.x;
@@ -304,52 +401,73 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
 // ---------------------------------------------------------------------------
 // the node forest: one tile's nodes per workgroup of kNodeCap threads
 // ---------------------------------------------------------------------------
+// With node_up: 1 + the hops of each exit's receiver to its tail (the
+// levels a node adds to its parent's).
 __global__ __launch_bounds__(kNodeCap) void k_deep_links(
-    const uint8_t *__restrict__ rcv, const uint32_t *__restrict__ link,
-    const int32_t *__restrict__ node_cell, const int32_t *__restrict__ nexit,
-    uint32_t *__restrict__ node_parent, uint32_t *__restrict__ node_kids, int W) {
-  const int t = blockIdx.x, k = threadIdx.x;
+    GraphWork w, const uint8_t *__restrict__ rcv0, const uint32_t *__restrict__ link0,
+    const uint32_t *__restrict__ dinfo0, const int32_t *__restrict__ node_cell,
+    const int32_t *__restrict__ nexit, uint32_t *__restrict__ node_parent,
+    uint32_t *__restrict__ node_kids, uint32_t *__restrict__ node_up) {
+  const int k = threadIdx.x, wi = blockIdx.x;
+  if (wi >= w.size[1]) return;
+  const WorkTile wt = work_tile(w, wi);
+  const int t = wt.tg;
   if (k >= nexit[t]) return;
   const int id = t * kNodeCap + k;
   const int32_t g = node_cell[id];
   // the receiver lies on the border of the neighbouring tile
-  const uint32_t p = link[g + dir_offset(rcv[g], W)];
+  const int32_t y = g + dir_offset(rcv0[wt.off + g], wt.W);
+  const uint32_t p = link0[wt.off + y];
   node_parent[id] = p;
   if (p != kNoNode) atomicAdd(&node_kids[p], 1u);
+  if (node_up) node_up[id] = 1u + static_cast<uint32_t>(info_hops(dinfo0[wt.off + y]));
 }
 
 // Tier of every node, by walking its parents (a chain of tiles crossed by
 // one drainage path); per tier, the number of tiles holding cells of it.
 __global__ __launch_bounds__(kNodeCap) void k_deep_tiers(
-    const uint32_t *__restrict__ node_parent, const int32_t *__restrict__ nexit,
-    uint16_t *__restrict__ node_tier, int32_t *__restrict__ thist, int32_t *__restrict__ tstat) {
+    GraphWork w, const uint32_t *__restrict__ node_parent, const int32_t *__restrict__ nexit,
+    uint16_t *__restrict__ node_tier, const uint32_t *__restrict__ node_up,
+    uint32_t *__restrict__ node_level, int32_t *__restrict__ thist,
+    int32_t *__restrict__ tstat) {
   __shared__ uint32_t seen[(kTierCap + 32) / 32];   // tiers already counted for the tile
-  const int t = blockIdx.x, k = threadIdx.x;
-  const int n = nexit[t];
-  int tier = 0;
-  if (k < n) {
-    const int id = t * kNodeCap + k;
-    tier = 1;
-    uint32_t p = node_parent[id];
-    while (p != kNoNode && tier <= kTierCap) {
-      tier++;
-      p = node_parent[p];
+  const int k = threadIdx.x, wi = blockIdx.x;
+  if (wi >= w.size[1]) return;
+  {
+    const int t = work_tile(w, wi).tg;
+    const int n = nexit[t];
+    int tier = 0;
+    if (k < n) {
+      const int id = t * kNodeCap + k;
+      tier = 1;
+      uint32_t lev = node_up ? node_up[id] : 0u;
+      uint32_t p = node_parent[id];
+      while (p != kNoNode && tier <= kTierCap) {
+        tier++;
+        if (node_up) lev += node_up[p];
+        p = node_parent[p];
+      }
+      node_tier[id] = static_cast<uint16_t>(min(tier, kTierCap));
+      if (node_level) node_level[id] = lev;
     }
-    node_tier[id] = static_cast<uint16_t>(min(tier, kTierCap));
+    // clear the words the tile's tiers can touch (the block's own entries)
+    if (k < n && tier <= kTierCap) seen[tier >> 5] = 0;
+    __syncthreads();
+    if (k < n) {
+      if (tier > kTierCap) {
+        atomicOr(&tstat[kStErr], kErrTier);
+      } else {
+        const uint32_t bit = 1u << (tier & 31);
+        if (!(atomicOr(&seen[tier >> 5], bit) & bit)) {   // the tile's first node of the tier
+          atomicAdd(&thist[tier * kShards + (t & (kShards - 1))], 1);
+          if (tier > __hip_atomic_load(&tstat[kStMaxTier], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&tstat[kStMaxTier], tier);
+        }
+      }
+    }
+    __syncthreads();
   }
-  // clear the words the tile's tiers can touch (the block's own entries)
-  if (k < n && tier <= kTierCap) seen[tier >> 5] = 0;
-  __syncthreads();
-  if (k >= n) return;
-  if (tier > kTierCap) {
-    atomicOr(&tstat[kStErr], kErrTier);
-    return;
-  }
-  const uint32_t bit = 1u << (tier & 31);
-  if (atomicOr(&seen[tier >> 5], bit) & bit) return;   // another node of the tile counts it
-  atomicAdd(&thist[tier * kShards + (t & (kShards - 1))], 1);
-  if (tier > __hip_atomic_load(&tstat[kStMaxTier], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-    atomicMax(&tstat[kStMaxTier], tier);
 }
 
 // Offsets of the work items by tier: thist (counts per tier and shard) ->
@@ -381,24 +499,104 @@ __global__ __launch_bounds__(kBlock) void k_deep_scan(int32_t *__restrict__ this
 
 // Count through every exit, leaf to root.
 __global__ __launch_bounds__(kNodeCap) void k_deep_sum(
-    uint64_t *__restrict__ node_word, const uint32_t *__restrict__ node_kids,
+    GraphWork w, uint64_t *__restrict__ node_word, const uint32_t *__restrict__ node_kids,
     const uint32_t *__restrict__ node_parent, const int32_t *__restrict__ nexit,
-    uint32_t *__restrict__ node_out, int32_t *__restrict__ tstat, int nnodes) {
-  const int t = blockIdx.x, k = threadIdx.x;
-  if (k >= nexit[t]) return;
-  uint32_t id = static_cast<uint32_t>(t * kNodeCap + k);
-  if (node_kids[id] != 0) return;          // summed by its last child's thread
-  uint64_t val = node_word[id] & kSumMask;
-  for (int guard = 0; guard <= nnodes; guard++) {
-    node_out[id] = static_cast<uint32_t>(val);
-    const uint32_t p = node_parent[id];
-    if (p == kNoNode) return;
-    const uint64_t old = atomicAdd(&node_word[p], val + kOneArrival);
-    if ((old >> kSumShift) + 1 != node_kids[p]) return;   // not the last child
-    val += old & kSumMask;
-    id = p;
+    uint32_t *__restrict__ node_out, int32_t *__restrict__ tstat) {
+  const int k = threadIdx.x, wi = blockIdx.x;
+  if (wi >= w.size[1]) return;
+  {
+    const int t = work_tile(w, wi).tg;
+    if (k >= nexit[t]) return;
+    uint32_t id = static_cast<uint32_t>(t * kNodeCap + k);
+    if (node_kids[id] != 0) return;          // summed by its last child's thread
+    uint64_t val = node_word[id] & kSumMask;
+    bool ended = false;
+    for (int guard = 0; guard <= kTierCap; guard++) {
+      node_out[id] = static_cast<uint32_t>(val);
+      const uint32_t p = node_parent[id];
+      if (p == kNoNode) {
+        ended = true;
+        break;
+      }
+      const uint64_t old = atomicAdd(&node_word[p], val + kOneArrival);
+      if ((old >> kSumShift) + 1 != node_kids[p]) {   // not the last child
+        ended = true;
+        break;
+      }
+      val += old & kSumMask;
+      id = p;
+    }
+    if (!ended) atomicOr(&tstat[kStErr], kErrChain);
   }
-  atomicOr(&tstat[kStErr], kErrChain);
+}
+
+// ---------------------------------------------------------------------------
+// k_deep_levels: per tile, each cell's level (hops to its root across tiles:
+// its hops to its tail, plus its tail node's levels) counted into its grid's
+// level histogram, and the grid's deepest level; for grids that keep them
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_deep_levels(GraphWork w,
+                                                        const uint32_t *__restrict__ dinfo0,
+                                                        const uint32_t *__restrict__ link0,
+                                                        const uint32_t *__restrict__ node_level) {
+  __shared__ int hist[kCells];
+  __shared__ int lo_s, hi_s;
+  const int wi = blockIdx.x;
+  if (wi >= w.size[1]) return;
+  {
+    const WorkTile wt = work_tile(w, wi);
+    const WorkTile &G = wt;
+    if (!G.levels) return;   // block-uniform
+    const Box b = tile_box(wt.t, G.tiles_x, G.W, G.H);
+    if (threadIdx.x == 0) {
+      lo_s = 0x7FFFFFFF;
+      hi_s = 0;
+    }
+    __syncthreads();
+    int lev[kPer];
+    int lo = 0x7FFFFFFF, hi = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const int i = j * kBlock + threadIdx.x;
+      const int lx = i % kSide, ly = i / kSide;
+      lev[j] = -1;
+      if (lx >= b.tw || ly >= b.th) continue;
+      const uint32_t inf = dinfo0[G.off + static_cast<size_t>(b.y0 + ly) * G.W + b.x0 + lx];
+      int L = info_hops(inf);
+      if (inf & kInfoExit) {
+        const int tl = info_tail(inf);
+        const size_t tg = static_cast<size_t>(b.y0 + tl / kSide) * G.W + b.x0 + tl % kSide;
+        L += static_cast<int>(node_level[link0[G.off + tg]]);   // an exit's link is its node
+      }
+      lev[j] = L;
+      lo = min(lo, L);
+      hi = max(hi, L);
+    }
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      lo = min(lo, __shfl_xor(lo, d));
+      hi = max(hi, __shfl_xor(hi, d));
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+      atomicMin(&lo_s, lo);
+      atomicMax(&hi_s, hi);
+    }
+    __syncthreads();
+    const int base = lo_s, span = min(hi_s - lo_s + 1, kCells);
+    for (int q = threadIdx.x; q < span; q += kBlock) hist[q] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      if (lev[j] < 0 || lev[j] >= G.nlev) continue;
+      if (lev[j] - base < span) atomicAdd(&hist[lev[j] - base], 1);
+      else atomicAdd(&G.levels[lev[j]], 1);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < span; q += kBlock)
+      if (hist[q] && base + q < G.nlev) atomicAdd(&G.levels[base + q], hist[q]);
+    if (threadIdx.x == 0) atomicMax(&G.hdr[0], hi_s);
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -429,18 +627,29 @@ __device__ __forceinline__ int tier_rank(const int *sorted, int n, int tier) {
 
 template <int NMODE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_deep_counts(
-    const uint8_t *__restrict__ rcv, uint32_t *__restrict__ cnt,
-    const uint32_t *__restrict__ dinfo, const uint32_t *__restrict__ link,
+    GraphWork w, const uint8_t *__restrict__ rcv0, uint32_t *__restrict__ cnt0,
+    const uint32_t *__restrict__ dinfo0, const uint32_t *__restrict__ link0,
     const uint32_t *__restrict__ node_out, const int32_t *__restrict__ node_cell,
     const uint16_t *__restrict__ node_tier, const int32_t *__restrict__ nexit,
     const int32_t *__restrict__ tile_n0, uint16_t *__restrict__ cells,
-    int32_t *__restrict__ tcur, uint64_t *__restrict__ items, const double *__restrict__ ftab,
-    const double *__restrict__ h_old, double *__restrict__ h_new, int W, int H, int tiles_x,
-    ErodeConst k, int32_t *hdr) {
+    int32_t *__restrict__ tcur, uint64_t *__restrict__ items, const double *__restrict__ ftab0,
+    const double *__restrict__ h_old0, double *__restrict__ h_new0) {
   __shared__ CountShared sh;
-  const int t = blockIdx.x;
-  const Box b = tile_box(t, tiles_x, W, H);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int wi = blockIdx.x;   // one tile per workgroup; the grid covers the list exactly
+  if (wi >= w.size[1]) return;
+  const WorkTile wt = work_tile(w, wi);
+  const WorkTile &G = wt;
+  const int W = G.W, H = G.H;
+  const uint8_t *const rcv = rcv0 + G.off;
+  uint32_t *const cnt = cnt0 + G.off;
+  const uint32_t *const dinfo = dinfo0 + G.off, *const link = link0 + G.off;
+  const double *const ftab = ftab0 + G.ftab_off, *const h_old = h_old0 + G.off;
+  double *const h_new = h_new0 + G.off;
+  const ErodeConst k = G.ek;
+  int32_t *const hdr = G.hdr;
+  const int t = wt.tg;   // per-tile arrays and node ids
+  const Box b = tile_box(wt.t, G.tiles_x, W, H);
   uint32_t info[kPer];
   if (threadIdx.x == 0) sh.ntier = sh.maxhops = 0;
   if (threadIdx.x < kTiersPerTile) sh.hist[threadIdx.x] = 0;
@@ -556,7 +765,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     if (static_cast<int>(threadIdx.x) < ntier) {
       const int tier = sh.tiers[threadIdx.x];
       const int pos = atomicAdd(&tcur[tier * kShards + (t & (kShards - 1))], 1);
-      items[pos] = item_pack(t, n0 + ex, v);
+      items[pos] = item_pack(wi, n0 + ex, v);
       sh.hist[threadIdx.x] = n0 + ex;   // first slot of the rank
     }
   }
@@ -634,7 +843,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
           const int gx = b.x0 + i % kSide, gy = b.y0 + i / kSide;
           if (gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2) hn += k.ud;
         } else {
-          hn = newton_update<NMODE>(h0 + k.ud, sh.hs[ri], fact, k, hdr);
+          hn = solve<NMODE>(G.nmode, h0 + k.ud, sh.hs[ri], fact, k, hdr);
         }
         sh.hs[i] = hn;
         __hip_atomic_store(&sh.done[i], static_cast<uint8_t>(1), __ATOMIC_RELEASE,
@@ -667,11 +876,10 @@ struct ErodeShared {
 // cell's receiver in the tile (one hop nearer the exit) is always earlier.
 template <int NMODE>
 __global__ __launch_bounds__(kEroBlock) void k_deep_erode(
-    const uint8_t *__restrict__ rcv, const uint32_t *__restrict__ cnt,
-    const uint32_t *__restrict__ dinfo, const uint16_t *__restrict__ cells,
-    const uint64_t *__restrict__ items, const double *__restrict__ ftab,
-    const double *__restrict__ h_old, double *h_new, int32_t *tstat, int W, int H, int tiles_x,
-    ErodeConst k, int32_t *hdr) {
+    GraphWork w, const uint8_t *__restrict__ rcv0, const uint32_t *__restrict__ cnt0,
+    const uint32_t *__restrict__ dinfo0, const uint16_t *__restrict__ cells,
+    const uint64_t *__restrict__ items, const double *__restrict__ ftab0,
+    const double *__restrict__ h_old0, double *h_new0, int32_t *tstat) {
   __shared__ ErodeShared sh;
   const int nitems = tstat[kStItems];
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -688,11 +896,19 @@ __global__ __launch_bounds__(kEroBlock) void k_deep_erode(
     __syncthreads();
     const uint64_t item = sh.item;
     if (item == ~uint64_t{0}) return;
-    const int t = static_cast<int>(item & ((1u << 27) - 1));
+    const WorkTile wt = work_tile(w, static_cast<int>(item & ((1u << 27) - 1)));
+    const WorkTile &G = wt;
     const int first = static_cast<int>((item >> 27) & 0x1FFF);
     const int n = static_cast<int>(item >> 40);
-    const Box b = tile_box(t, tiles_x, W, H);
-    const uint16_t *const tl = cells + static_cast<size_t>(t) * kCells + first;
+    const int W = G.W;
+    const uint8_t *const rcv = rcv0 + G.off;
+    const uint32_t *const cnt = cnt0 + G.off, *const dinfo = dinfo0 + G.off;
+    const double *const ftab = ftab0 + G.ftab_off, *const h_old = h_old0 + G.off;
+    double *const h_new = h_new0 + G.off;
+    const ErodeConst k = G.ek;
+    int32_t *const hdr = G.hdr;
+    const Box b = tile_box(wt.t, G.tiles_x, W, G.H);
+    const uint16_t *const tl = cells + static_cast<size_t>(wt.tg) * kCells + first;
     for (int p = threadIdx.x; p < n; p += kEroBlock) sh.done[tl[p]] = 0;
     __syncthreads();
     // wavefront: each wave takes 64-cell chunks in order; a lane solves its
@@ -731,7 +947,7 @@ __global__ __launch_bounds__(kEroBlock) void k_deep_erode(
         if (ready || give_up) {
           double hn = h0;
           if (give_up) atomicOr(&hdr[2], kErrLookback);
-          else hn = newton_update<NMODE>(h0 + k.ud, hr, fact, k, hdr);
+          else hn = solve<NMODE>(G.nmode, h0 + k.ud, hr, fact, k, hdr);
           sh.hs[i] = hn;
           __hip_atomic_store(&sh.done[i], static_cast<uint8_t>(1), __ATOMIC_RELEASE,
                              __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -757,5 +973,6 @@ __global__ void k_deep_clear(int32_t *tstat) {
 }
 
 }  // namespace deep
+}  // namespace
 
 #endif  // FS_DEEP_H

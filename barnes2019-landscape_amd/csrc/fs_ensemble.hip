@@ -13,20 +13,21 @@
 //   A  tile accumulation (upstream counts and levels, RB:213-279) of every
 //      candidate's 64x64 tiles; a tile deeper than kHalo marks its
 //      realization deep
-//   -  block 0 lists the erosion work: deep realizations first, largest
-//      first, each as a group of workgroups; then the shallow ones' tiles
-//   E  work items claimed in list order from one counter:
-//        shallow: tile uplift + erosion in LDS (RB:285-328)
-//        deep:    a level path run by the realization's workgroup group:
-//                 donor masks and outlets, BFS levels (RB:213-254) that
-//                 record each stack entry's receiver and donor positions,
-//                 counts pulled from the top level down (RB:262-279) and
-//                 uplift and erosion level by level (RB:285-328) in stack
-//                 order, with group barriers between levels
+//   -  block 0 lists the erosion work (the shallow realizations' tiles) and
+//      the deep realizations (a tile deeper than kHalo, or deeper than kHalo
+//      last step) for the tile-graph kernels
+//   E  work items claimed in list order from one counter: tile uplift +
+//      erosion in LDS (RB:285-328)
 //
-// Results do not depend on the schedule (work item sizes, group sizes): each
-// cell's elevation depends only on its own inputs and its receiver's final
-// value, and upstream counts are integers.
+// The deep realizations then take one tile-graph step together (fs_deep.h):
+// per 64x64 tile with no halo, every deep realization's tiles in the same
+// launches, the tiles joined by the graph of the cells where drainage leaves
+// a tile; the step also counts their levels, so the next step knows which
+// are shallow again.
+//
+// Results do not depend on the schedule (work item sizes): each cell's
+// elevation depends only on its own inputs and its receiver's final value,
+// and upstream counts are integers.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,13 +46,13 @@
 #include "fs_hostutil.h"
 
 #include "fs_device.h"
+#include "fs_deep.h"
 
 namespace {
 
 constexpr int kEnsBlock = 1024;
 constexpr int kRecvLanes = 256;                     // columns per receiver work item
 constexpr int kRecvQuarters = kEnsBlock / kRecvLanes;
-constexpr int kMaxGroup = 16;                       // workgroups per deep realization
 constexpr int kHistLen = kHalo + 1;
 constexpr int kSpinCap = 1 << 22;                   // bounded waits (seconds)
 constexpr int kMaxRealizations = 1 << 20;
@@ -61,26 +62,24 @@ constexpr int kRH = 16;
 constexpr int kRhDepth = 0;       // deepest level of the last step
 constexpr int kRhOver = 1;        // tiles deeper than kHalo this step
 constexpr int kRhErr = 2;         // error bits (same slot as a model's hdr[2])
-constexpr int kRhCtr = 3;         // 3 rotating BFS append counters (3..5)
-constexpr int kRhBar = 6;         // group barrier counter
-constexpr int kRhPath = 7;        // path of the last step: 1 tiles, 2 levels
+constexpr int kRhPath = 7;        // path of the last step: 1 tiles, 3 tile graph
 constexpr int kRhCand = 8;        // tile candidate this step
 constexpr int kRhTileSteps = 9;
-constexpr int kRhLevelSteps = 10;
-constexpr int kRhGroup = 11;      // workgroups of the level path this step
+constexpr int kRhLevelSteps = 10;   // steps off the tiles (the tile graph)
 
 // global control words
 constexpr int kCtlBarrier = 0, kCtlClaim = 1, kCtlAbort = 2;
 constexpr int kCtlEntriesA = 3, kCtlItemsA = 4, kCtlEntriesE = 5, kCtlItemsE = 6;
-constexpr int kCtlInts = 8;
+constexpr int kCtlGraph = 8;      // the tile-graph work list's size: [8] entries, [9] tiles
+constexpr int kCtlInts = 10;
 
 struct EnsReal {
   uint64_t off;        // first cell of the realization in the cell arrays
   uint64_t ftab_off;   // first entry of its erosion factor table
-  uint64_t lev_off;    // first entry of its level offsets
+  uint64_t lev_off;    // first entry of its level counts
   uint64_t plan_off;   // first tile plan of the realization
   int W, H, tiles_x, ntiles;
-  int lev_cap;         // level offsets available (deepest level + 3)
+  int lev_cap;         // level entries available
   int nmode;
   int recv_xb;         // receiver column blocks
   int pad;
@@ -91,21 +90,16 @@ struct EnsReal {
 struct EnsArgs {
   const EnsReal *real;
   int R;
-  int recv_rows, tiles_per_claim, group_cells;
+  int recv_rows, tiles_per_claim;
   double *h_old, *h_new;
-  uint8_t *rcv, *lvl, *dmask;
+  uint8_t *rcv, *lvl;
   uint32_t *cnt;
   TilePlan *plan;
-  int32_t *order, *lev;
-  // the level path's records in stack order (index = position in `order`)
-  uint8_t *rdir;      // receiver direction
-  int32_t *fdon;      // position of the first donor
-  uint8_t *ndon;      // number of donors
-  uint32_t *cnt_s;    // upstream cell count
   const double *ftab;
   int32_t *rh, *hist, *ctl;
   const int32_t *recv_first;   // R+1 prefix of receiver work items
   int32_t *tabA_off, *tabA_real, *tabE_off, *tabE_real;
+  int32_t *tabG_off, *tabG_real;   // the deep realizations (tile-graph work list)
   uint32_t bar_base;
 };
 
@@ -138,15 +132,6 @@ __device__ __forceinline__ void counter_barrier(int32_t *bar, uint32_t &target, 
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-__device__ __forceinline__ void group_barrier(int G, int32_t *bar, uint32_t &target,
-                                              int32_t *ctl) {
-  if (G == 1) {
-    __syncthreads();
-    return;
-  }
-  counter_barrier(bar, target, G, ctl);
-}
-
 // Reserve k slots per lane from *ctr with one atomic per wave; returns the
 // lane's first slot.  Every lane of the wave must call it.
 __device__ __forceinline__ int wave_reserve(int32_t *ctr, int k) {
@@ -175,10 +160,6 @@ __device__ __forceinline__ int find_entry(const int32_t *off, int n, int v) {
   return lo;
 }
 
-struct DeepShared {
-  int32_t ctr[3];
-};
-
 struct TableShared {
   uint32_t scan[kEnsBlock / kWave + 1];
 };
@@ -186,7 +167,6 @@ struct TableShared {
 union EnsShared {
   AccShared acc;
   EroShared ero;
-  DeepShared deep;
   TableShared tab;
 };
 
@@ -209,173 +189,6 @@ __device__ void append_entries(int R, Pick pick, Count count, int32_t *off, int3
     }
     entries += static_cast<int>(te);
     items += static_cast<int>(ti);
-  }
-}
-
-// The level path of one realization, run by its G workgroups (member is
-// this workgroup's rank).  Levels are discovered breadth first from the
-// outlets; a level's entries are appended in any order (the stack order of
-// GenerateOrder is not needed: counts are integers and each elevation
-// depends only on its receiver's), so every level's size -- the levels[]
-// offsets -- matches the reference's.  The breadth-first pass records, for
-// every stack position, its receiver direction and where its donors were
-// appended, so counts are pulled from contiguous runs and the erosion reads
-// count and direction contiguously; only the cell's elevation and its
-// receiver's (usually in the same cache line) are read from the grid.
-template <int NMODE>
-__device__ void deep_path(const EnsArgs &a, const EnsReal &g, int r, int member, int G,
-                          DeepShared &sd) {
-  const int W = g.W, H = g.H;
-  const int n = W * H;
-  const int T = G * kEnsBlock;
-  const int t = member * kEnsBlock + threadIdx.x;
-  int32_t *q = a.rh + static_cast<size_t>(r) * kRH;
-  const uint8_t *rc = a.rcv + g.off;
-  uint8_t *dm = a.dmask + g.off;
-  int32_t *ord = a.order + g.off;
-  uint8_t *rdir = a.rdir + g.off;
-  int32_t *fdon = a.fdon + g.off;
-  uint8_t *ndon = a.ndon + g.off;
-  uint32_t *cs = a.cnt_s + g.off;
-  const double *ho = a.h_old + g.off;
-  double *hn = a.h_new + g.off;
-  int32_t *lev = a.lev + g.lev_off;
-  const double *ft = a.ftab + g.ftab_off;
-  int32_t *ctr = G == 1 ? sd.ctr : q + kRhCtr;
-  uint32_t bar = 0;
-  if (G == 1) {
-    if (threadIdx.x < 3) sd.ctr[threadIdx.x] = 0;
-    __syncthreads();
-  }
-  // donor masks (the inverse of the receivers, RB:188-203) and level 0, the
-  // cells without a receiver (RB:223-226)
-  for (int base = 0; base < n; base += T) {
-    const int c = base + t;
-    bool root = false;
-    if (c < n) {
-      const int x = c % W, y = c / W;
-      root = rc[c] >= 8;
-      // slot j holds the neighbour at {-W-1,-W,-W+1,-1,+1,W-1,W,W+1}[j]; it is a
-      // donor when its receiver points back at c.  Ring 0 is never a receiver.
-      uint32_t m = 0;
-      if (x > 0 && x < W - 1 && y > 0 && y < H - 1) {
-        const uint8_t *up = rc + c - W, *dn = rc + c + W;
-        m |= (up[-1] == 5) ? 0x01u : 0u;
-        m |= (up[0] == 6) ? 0x02u : 0u;
-        m |= (up[1] == 7) ? 0x04u : 0u;
-        m |= (rc[c - 1] == 4) ? 0x08u : 0u;
-        m |= (rc[c + 1] == 0) ? 0x10u : 0u;
-        m |= (dn[-1] == 3) ? 0x20u : 0u;
-        m |= (dn[0] == 2) ? 0x40u : 0u;
-        m |= (dn[1] == 1) ? 0x80u : 0u;
-      }
-      dm[c] = static_cast<uint8_t>(m);
-    }
-    const int pos = wave_reserve(ctr + 0, root ? 1 : 0);
-    if (root) {
-      if (pos < n) ord[pos] = c;
-      else atomicOr(&q[kRhErr], kErrIndex);
-    }
-  }
-  if (t == 0 && G > 1) __hip_atomic_store(ctr + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  group_barrier(G, q + kRhBar, bar, a.ctl);
-  // breadth-first levels: level L is ord[lo, hi); its donors become level
-  // L+1, each donor list contiguous from fdon[p]
-  int lo = 0, hi = ld_agent(ctr + 0);
-  if (t == 0) lev[0] = 0;
-  int L = 0;
-  bool failed = false;
-  for (;;) {
-    if (L + 3 > g.lev_cap) {
-      if (t == 0) atomicOr(&q[kRhErr], kErrDepth);
-      failed = true;
-      break;
-    }
-    if (t == 0) lev[L + 1] = hi;
-    int32_t *cl = ctr + (L + 1) % 3;
-    if (t == 0) __hip_atomic_store(ctr + (L + 2) % 3, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int base = lo; base < hi; base += T) {
-      const int p = base + t;
-      int c = 0;
-      uint32_t m = 0;
-      if (p < hi) {
-        c = ord[p];
-        if (static_cast<uint32_t>(c) < static_cast<uint32_t>(n)) m = dm[c];
-        else atomicOr(&q[kRhErr], kErrIndex);
-      }
-      const int k = __popc(m);
-      int pos = hi + wave_reserve(cl, k);
-      if (p < hi) {
-        fdon[p] = pos;
-        ndon[p] = static_cast<uint8_t>(k);
-      }
-      while (m) {
-        const int j = __builtin_ctz(m);
-        m &= m - 1;
-        if (pos < n) {
-          ord[pos] = c + slot_offset(j, W);
-          rdir[pos] = static_cast<uint8_t>(slot_dir(j));
-        } else {
-          atomicOr(&q[kRhErr], kErrIndex);
-        }
-        pos++;
-      }
-    }
-    group_barrier(G, q + kRhBar, bar, a.ctl);
-    const int next = hi + ld_agent(cl);
-    lo = hi;
-    hi = next;
-    if (hi == lo || hi > n) break;
-    L++;
-  }
-  const int D = L;   // levels 0..D are non-empty
-  if (failed || hi != n) {
-    if (t == 0 && !failed) atomicOr(&q[kRhErr], kErrIndex);
-    return;   // every member leaves together: the values above are shared
-  }
-  // upstream cell counts, top level first, pulled from each entry's
-  // contiguous donors (pull form of RB:262-279)
-  for (int l = D; l >= 0; l--) {
-    const int ls = lev[l], le = lev[l + 1];
-    for (int p = ls + t; p < le; p += T) {
-      const int f = fdon[p], k = ndon[p];
-      uint32_t s = 1;
-      if (f >= le && f + k <= n) {
-        for (int i = 0; i < k; i++) s += cs[f + i];
-      } else if (k > 0) {
-        atomicOr(&q[kRhErr], kErrIndex);
-      }
-      cs[p] = s;
-    }
-    group_barrier(G, q + kRhBar, bar, a.ctl);
-  }
-  // uplift (RB:285-296) and erosion (RB:306-328), outlets first; each
-  // entry's receiver is final one level down
-  const ErodeConst k = g.ek;
-  for (int l = 0; l <= D; l++) {
-    const int ls = lev[l], le = lev[l + 1];
-    for (int p = ls + t; p < le; p += T) {
-      const int c = ord[p];
-      if (l == 0) {
-        const int x = c % W, y = c / W;
-        const bool interior = x >= 2 && x < W - 2 && y >= 2 && y < H - 2;
-        hn[c] = interior ? ho[c] + k.ud : ho[c];
-      } else {
-        const int rr = rdir[p] & 7;
-        const int nb = c + dir_offset(rr, W);
-        uint32_t cc = cs[p];
-        if (cc > static_cast<uint32_t>(n) || static_cast<uint32_t>(nb) >= static_cast<uint32_t>(n)) {
-          atomicOr(&q[kRhErr], kErrIndex);
-          continue;
-        }
-        hn[c] = newton_update<NMODE>(ho[c] + k.ud, hn[nb], ft[2 * cc + (rr & 1)], k, q);
-      }
-    }
-    group_barrier(G, q + kRhBar, bar, a.ctl);
-  }
-  if (t == 0) {
-    lev[D + 2] = n;
-    q[kRhDepth] = D;
   }
 }
 
@@ -402,8 +215,6 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_step(EnsArgs a) {
   for (int r = blockIdx.x * kEnsBlock + threadIdx.x; r < R; r += gridDim.x * kEnsBlock) {
     int32_t *q = a.rh + static_cast<size_t>(r) * kRH;
     q[kRhOver] = 0;
-    q[kRhCtr] = q[kRhCtr + 1] = q[kRhCtr + 2] = 0;
-    q[kRhBar] = 0;
     for (int l = 0; l < kHistLen; l++) a.hist[static_cast<size_t>(r) * kHistLen + l] = 0;
   }
   if (blockIdx.x == 0) {
@@ -465,13 +276,8 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_step(EnsArgs a) {
     };
     for (int r = threadIdx.x; r < R; r += kEnsBlock) {
       int32_t *q = a.rh + static_cast<size_t>(r) * kRH;
-      const EnsReal &g = a.real[r];
       if (deep(r)) {
-        const int n = g.W * g.H;
-        int G = a.group_cells > 0 ? (n + a.group_cells - 1) / a.group_cells : 1;
-        G = max(1, min(G, min(kMaxGroup, static_cast<int>(gridDim.x))));
-        q[kRhGroup] = G;
-        q[kRhPath] = 2;
+        q[kRhPath] = 3;
         q[kRhLevelSteps]++;
       } else {
         int D = 0;
@@ -484,8 +290,15 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_step(EnsArgs a) {
     }
     __syncthreads();
     int entries = 0, items = 0;
-    append_entries(R, deep, [&](int r) { return a.rh[static_cast<size_t>(r) * kRH + kRhGroup]; },
-                   a.tabE_off, a.tabE_real, entries, items, sh.tab);
+    // the deep realizations' tiles: the tile-graph kernels' work list
+    append_entries(R, deep, [&](int r) { return a.real[r].ntiles; }, a.tabG_off, a.tabG_real,
+                   entries, items, sh.tab);
+    if (threadIdx.x == 0) {
+      a.tabG_off[entries] = items;
+      a.ctl[kCtlGraph] = entries;
+      a.ctl[kCtlGraph + 1] = items;
+    }
+    entries = items = 0;
     const int tpc = a.tiles_per_claim;
     append_entries(
         R, [&](int r) { return !deep(r); },
@@ -511,22 +324,17 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_step(EnsArgs a) {
     const EnsReal &g = a.real[r];
     int32_t *q = a.rh + static_cast<size_t>(r) * kRH;
     const int j = item - a.tabE_off[e];
-    if (q[kRhPath] == 2) {
-      const int G = q[kRhGroup];
-      with_nmode<NM>(g.nmode, [&](auto c) { deep_path<decltype(c)::value>(a, g, r, j, G, sh.deep); });
-    } else {
-      const int t1 = min(g.ntiles, (j + 1) * a.tiles_per_claim);
-      const ErodeConst ek = g.ek;
-      for (int tile = j * a.tiles_per_claim; tile < t1; tile++) {
-        with_nmode<NM>(g.nmode, [&](auto c) {
-          tile_erode_body<decltype(c)::value>(sh.ero, a.h_old + g.off, a.h_new + g.off,
-                                              a.plan + g.plan_off + tile, a.lvl + g.off,
-                                              a.cnt + g.off,
-                                              a.ftab + g.ftab_off, g.W, g.H, tile, g.tiles_x,
-                                              ek, q);
-        });
-        __syncthreads();
-      }
+    const int t1 = min(g.ntiles, (j + 1) * a.tiles_per_claim);
+    const ErodeConst ek = g.ek;
+    for (int tile = j * a.tiles_per_claim; tile < t1; tile++) {
+      with_nmode<NM>(g.nmode, [&](auto c) {
+        tile_erode_body<decltype(c)::value>(sh.ero, a.h_old + g.off, a.h_new + g.off,
+                                            a.plan + g.plan_off + tile, a.lvl + g.off,
+                                            a.cnt + g.off,
+                                            a.ftab + g.ftab_off, g.W, g.H, tile, g.tiles_x,
+                                            ek, q);
+      });
+      __syncthreads();
     }
     __syncthreads();
   }
@@ -552,17 +360,26 @@ struct fs_ensemble {
 
   EnsReal *real_d = nullptr;
   double *h[2] = {nullptr, nullptr};
-  uint8_t *rcv = nullptr, *lvl = nullptr, *dmask = nullptr;
+  uint8_t *rcv = nullptr, *lvl = nullptr;
   uint32_t *cnt = nullptr;
   TilePlan *plan = nullptr;
-  int32_t *order = nullptr, *lev = nullptr;
-  uint8_t *rdir = nullptr;                    // level path, stack order
-  int32_t *fdon = nullptr;
-  uint8_t *ndon = nullptr;
-  uint32_t *cnt_s = nullptr;
+  int32_t *lev = nullptr;                     // per-level counts of tile-graph steps
   double *ftab = nullptr;
   int32_t *rh = nullptr, *hist = nullptr, *ctl = nullptr;
   int32_t *tabA_off = nullptr, *tabA_real = nullptr, *tabE_off = nullptr, *tabE_real = nullptr;
+  int32_t *tabG_off = nullptr, *tabG_real = nullptr;
+  int32_t *graph_n_h = nullptr;               // pinned: the tile-graph work list's size
+  // the tile-graph step of the deep realizations (fs_deep.h)
+  deep::GraphGrid *graph_grid = nullptr;
+  uint32_t *dinfo = nullptr, *link = nullptr;
+  uint64_t *node_word = nullptr, *items = nullptr;
+  uint32_t *node_kids = nullptr, *node_parent = nullptr, *node_out = nullptr;
+  uint32_t *node_up = nullptr, *node_level = nullptr;
+  int32_t *node_cell = nullptr, *nexit = nullptr, *tile_n0 = nullptr;
+  uint16_t *node_tier = nullptr, *cells_list = nullptr;
+  int32_t *thist = nullptr, *tcur = nullptr, *tstat = nullptr;
+  int graph_blocks = 0;
+  uint64_t graph_tiles = 0;                   // tile-graph tiles stepped so far
   std::map<int, int32_t *> recv_first;   // per recv_rows
   int cur = 0;
   uint32_t bar_base = 0;
@@ -586,17 +403,24 @@ void free_ensemble(fs_ensemble *e) {
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   for (void *p : {static_cast<void *>(e->real_d), static_cast<void *>(e->h[0]),
                   static_cast<void *>(e->h[1]), static_cast<void *>(e->rcv),
-                  static_cast<void *>(e->lvl), static_cast<void *>(e->dmask),
-                  static_cast<void *>(e->cnt), static_cast<void *>(e->plan),
-                  static_cast<void *>(e->order), static_cast<void *>(e->rdir),
-                  static_cast<void *>(e->fdon), static_cast<void *>(e->ndon),
-                  static_cast<void *>(e->cnt_s),
-                  static_cast<void *>(e->lev), static_cast<void *>(e->ftab),
-                  static_cast<void *>(e->rh), static_cast<void *>(e->hist),
-                  static_cast<void *>(e->ctl), static_cast<void *>(e->tabA_off),
-                  static_cast<void *>(e->tabA_real), static_cast<void *>(e->tabE_off),
-                  static_cast<void *>(e->tabE_real)})
+                  static_cast<void *>(e->lvl), static_cast<void *>(e->cnt),
+                  static_cast<void *>(e->plan), static_cast<void *>(e->lev),
+                  static_cast<void *>(e->ftab), static_cast<void *>(e->rh),
+                  static_cast<void *>(e->hist), static_cast<void *>(e->ctl),
+                  static_cast<void *>(e->tabA_off), static_cast<void *>(e->tabA_real),
+                  static_cast<void *>(e->tabE_off), static_cast<void *>(e->tabE_real),
+                  static_cast<void *>(e->tabG_off), static_cast<void *>(e->tabG_real),
+                  static_cast<void *>(e->graph_grid), static_cast<void *>(e->dinfo),
+                  static_cast<void *>(e->link), static_cast<void *>(e->node_word),
+                  static_cast<void *>(e->items), static_cast<void *>(e->node_kids),
+                  static_cast<void *>(e->node_parent), static_cast<void *>(e->node_out),
+                  static_cast<void *>(e->node_up), static_cast<void *>(e->node_level),
+                  static_cast<void *>(e->node_cell), static_cast<void *>(e->nexit),
+                  static_cast<void *>(e->tile_n0), static_cast<void *>(e->node_tier),
+                  static_cast<void *>(e->cells_list), static_cast<void *>(e->thist),
+                  static_cast<void *>(e->tcur), static_cast<void *>(e->tstat)})
     if (p) (void)hipFree(p);
+  if (e->graph_n_h) (void)hipHostFree(e->graph_n_h);
   for (auto &kv : e->recv_first)
     if (kv.second) (void)hipFree(kv.second);
   for (auto &ev : e->ev)
@@ -619,7 +443,7 @@ int check_schedule(const fs_schedule &s) {
   if (s.recv_rows < kRecvRows || s.recv_rows > 256 || s.recv_rows % kRecvRows != 0 ||
       s.tiles_per_claim < 1 || s.tiles_per_claim > 16 || s.group_cells < 0)
     return FS_ERR_INVALID_ARGUMENT;
-  return FS_OK;
+  return FS_OK;   // group_cells is accepted and has no effect (no level path any more)
 }
 
 // Receiver work items of every realization for `rows` rows per item.
@@ -656,6 +480,60 @@ int launch_step_nm(fs_ensemble *e, const EnsArgs &a) {
   return FS_OK;
 }
 
+// The tile-graph step of the realizations the step kernel listed as deep
+// (fs_deep.h), on the same stream.  The host reads the list's tile count
+// (one wait per step) so every per-tile kernel is launched with exactly one
+// workgroup per tile.
+template <int NM>
+int graph_launches(fs_ensemble *e, const deep::GraphWork &w, int T) {
+  hipStream_t s = e->stream;
+  const double *h_old = e->h[e->cur];
+  double *h_new = e->h[e->cur ^ 1];
+  hipLaunchKernelGGL(deep::k_deep_clear, dim3(1), dim3(64), 0, s, e->tstat);
+  hipLaunchKernelGGL(deep::k_deep_local, dim3(T), dim3(deep::kBlock), 0, s, w, e->rcv, e->cnt,
+                     e->dinfo, e->link, e->node_word, e->node_kids, e->node_cell, e->nexit,
+                     e->tile_n0, e->cells_list, h_new, e->tstat);
+  hipLaunchKernelGGL(deep::k_deep_links, dim3(T), dim3(deep::kNodeCap), 0, s, w, e->rcv, e->link,
+                     e->dinfo, e->node_cell, e->nexit, e->node_parent, e->node_kids, e->node_up);
+  hipLaunchKernelGGL(deep::k_deep_tiers, dim3(T), dim3(deep::kNodeCap), 0, s, w, e->node_parent,
+                     e->nexit, e->node_tier, e->node_up, e->node_level, e->thist, e->tstat);
+  hipLaunchKernelGGL(deep::k_deep_scan, dim3(1), dim3(deep::kBlock), 0, s, e->thist, e->tcur,
+                     e->tstat);
+  hipLaunchKernelGGL(deep::k_deep_sum, dim3(T), dim3(deep::kNodeCap), 0, s, w, e->node_word,
+                     e->node_kids, e->node_parent, e->nexit, e->node_out, e->tstat);
+  hipLaunchKernelGGL(deep::k_deep_levels, dim3(T), dim3(deep::kBlock), 0, s, w, e->dinfo, e->link,
+                     e->node_level);
+  hipLaunchKernelGGL(deep::k_deep_counts<NM>, dim3(T), dim3(deep::kBlock), 0, s, w, e->rcv,
+                     e->cnt, e->dinfo, e->link, e->node_out, e->node_cell, e->node_tier, e->nexit,
+                     e->tile_n0, e->cells_list, e->tcur, e->items, e->ftab, h_old, h_new);
+  hipLaunchKernelGGL(deep::k_deep_erode<NM>, dim3(static_cast<unsigned>(e->graph_blocks)),
+                     dim3(deep::kEroBlock), 0, s, w, e->rcv, e->cnt, e->dinfo, e->cells_list,
+                     e->items, e->ftab, h_old, h_new, e->tstat);
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    fs::set_last_error(std::string("tile-graph launch: ") + hipGetErrorString(err));
+    return FS_ERR_HIP;
+  }
+  return FS_OK;
+}
+
+int graph_step(fs_ensemble *e) {
+  FS_HIP_TRY(hipMemcpyAsync(e->graph_n_h, e->ctl + kCtlGraph, 2 * sizeof(int32_t),
+                            hipMemcpyDeviceToHost, e->stream));
+  FS_HIP_TRY(hipStreamSynchronize(e->stream));
+  const int T = e->graph_n_h[1];
+  if (T <= 0) return FS_OK;
+  deep::GraphWork w;
+  w.grid = e->graph_grid;
+  w.first = e->tabG_off;
+  w.which = e->tabG_real;
+  w.size = e->ctl + kCtlGraph;
+  e->graph_tiles += static_cast<uint64_t>(T);
+  if (e->nm_all == 1) return graph_launches<1>(e, w, T);
+  if (e->nm_all == 2) return graph_launches<2>(e, w, T);
+  return graph_launches<-1>(e, w, T);
+}
+
 int launch_step(fs_ensemble *e) {
   const int32_t *rf = nullptr;
   FS_TRY(recv_table(e, e->sched.recv_rows, &rf));
@@ -664,20 +542,12 @@ int launch_step(fs_ensemble *e) {
   a.R = e->R;
   a.recv_rows = e->sched.recv_rows;
   a.tiles_per_claim = e->sched.tiles_per_claim;
-  a.group_cells = e->sched.group_cells;
   a.h_old = e->h[e->cur];
   a.h_new = e->h[e->cur ^ 1];
   a.rcv = e->rcv;
   a.lvl = e->lvl;
-  a.dmask = e->dmask;
   a.cnt = e->cnt;
   a.plan = e->plan;
-  a.order = e->order;
-  a.rdir = e->rdir;
-  a.fdon = e->fdon;
-  a.ndon = e->ndon;
-  a.cnt_s = e->cnt_s;
-  a.lev = e->lev;
   a.ftab = e->ftab;
   a.rh = e->rh;
   a.hist = e->hist;
@@ -687,11 +557,14 @@ int launch_step(fs_ensemble *e) {
   a.tabA_real = e->tabA_real;
   a.tabE_off = e->tabE_off;
   a.tabE_real = e->tabE_real;
+  a.tabG_off = e->tabG_off;
+  a.tabG_real = e->tabG_real;
   a.bar_base = e->bar_base;
   int st;
   if (e->nm_all == 1) st = launch_step_nm<1>(e, a);
   else if (e->nm_all == 2) st = launch_step_nm<2>(e, a);
   else st = launch_step_nm<-1>(e, a);
+  if (st == FS_OK) st = graph_step(e);
   if (st != FS_OK) {
     e->broken = true;
     return st;
@@ -710,8 +583,17 @@ int ensemble_errors(fs_ensemble *e) {
   FS_HIP_TRY(hipMemcpy(ctl, e->ctl, sizeof(ctl), hipMemcpyDeviceToHost));
   if (ctl[kCtlAbort]) {
     e->broken = true;
-    fs::set_last_error("a grid or group barrier timed out");
+    fs::set_last_error("a grid barrier timed out");
     return FS_ERR_INTERNAL;
+  }
+  int32_t ts[deep::kStWords];
+  FS_HIP_TRY(hipMemcpy(ts, e->tstat, sizeof(ts), hipMemcpyDeviceToHost));
+  if (ts[deep::kStErr]) {
+    e->broken = true;
+    fs::set_last_error(ts[deep::kStErr] & deep::kErrTier
+                           ? "tile-graph step: a drainage path crosses more tile borders than levels allow"
+                           : "tile-graph step: receivers do not form a forest");
+    return ts[deep::kStErr] & deep::kErrTier ? FS_ERR_LEVEL_OVERFLOW : FS_ERR_INTERNAL;
   }
   std::vector<int32_t> rh(static_cast<size_t>(e->R) * kRH);
   FS_HIP_TRY(hipMemcpy(rh.data(), e->rh, rh.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -751,13 +633,12 @@ int timed_steps(fs_ensemble *e, const fs_schedule &s, int steps, double *ms_per_
 // Returns the number of steps taken in *used.
 int autotune(fs_ensemble *e, int budget, int *used) {
   *used = 0;
-  const std::vector<int> group_vals = {0, 1 << 22, 1 << 21, 1 << 20};
   const std::vector<int> rows_vals = {16, 64, 256};
   const std::vector<int> tpc_vals = {1, 2, 4};
   fs_schedule best = e->sched;
   double best_ms = -1;
-  for (int knob = 0; knob < 3; knob++) {
-    const std::vector<int> &vals = knob == 0 ? group_vals : knob == 1 ? rows_vals : tpc_vals;
+  for (int knob = 1; knob < 3; knob++) {
+    const std::vector<int> &vals = knob == 1 ? rows_vals : tpc_vals;
     for (int v : vals) {
       if (*used + e->tune_trial > budget) {
         e->sched = best;
@@ -918,14 +799,8 @@ int fs_ensemble_create(const fs_realization *reals, int count, int device, fs_en
   FS_TRY(fs::dev_alloc(&e->h[1], cells));
   FS_TRY(fs::dev_alloc(&e->rcv, cells + 16));
   FS_TRY(fs::dev_alloc(&e->lvl, cells));
-  FS_TRY(fs::dev_alloc(&e->dmask, cells));
   FS_TRY(fs::dev_alloc(&e->cnt, cells));
   FS_TRY(fs::dev_alloc(&e->plan, static_cast<size_t>(tiles_total)));
-  FS_TRY(fs::dev_alloc(&e->order, cells));
-  FS_TRY(fs::dev_alloc(&e->rdir, cells));
-  FS_TRY(fs::dev_alloc(&e->fdon, cells));
-  FS_TRY(fs::dev_alloc(&e->ndon, cells));
-  FS_TRY(fs::dev_alloc(&e->cnt_s, cells));
   FS_TRY(fs::dev_alloc(&e->lev, lev_n));
   FS_TRY(fs::dev_alloc(&e->ftab, ftab_n));
   FS_TRY(fs::dev_alloc(&e->rh, static_cast<size_t>(count) * kRH));
@@ -935,7 +810,62 @@ int fs_ensemble_create(const fs_realization *reals, int count, int device, fs_en
   FS_TRY(fs::dev_alloc(&e->tabA_real, count + 1));
   FS_TRY(fs::dev_alloc(&e->tabE_off, count + 1));
   FS_TRY(fs::dev_alloc(&e->tabE_real, count + 1));
+  FS_TRY(fs::dev_alloc(&e->tabG_off, count + 1));
+  FS_TRY(fs::dev_alloc(&e->tabG_real, count + 1));
+  // the tile-graph step (deep realizations): the realizations' tiles are
+  // numbered by plan_off, so the per-tile arrays cover every tile
+  {
+    const size_t T = static_cast<size_t>(tiles_total);
+    const size_t nodes = T * deep::kNodeCap;
+    const size_t tier_words = static_cast<size_t>(deep::kTierCap + 1) * deep::kShards;
+    FS_TRY(fs::dev_alloc(&e->graph_grid, count));
+    FS_TRY(fs::dev_alloc(&e->dinfo, cells));
+    FS_TRY(fs::dev_alloc(&e->link, cells));
+    FS_TRY(fs::dev_alloc(&e->node_word, nodes));
+    FS_TRY(fs::dev_alloc(&e->node_kids, nodes));
+    FS_TRY(fs::dev_alloc(&e->node_parent, nodes));
+    FS_TRY(fs::dev_alloc(&e->node_out, nodes));
+    FS_TRY(fs::dev_alloc(&e->node_up, nodes));
+    FS_TRY(fs::dev_alloc(&e->node_level, nodes));
+    FS_TRY(fs::dev_alloc(&e->node_cell, nodes));
+    FS_TRY(fs::dev_alloc(&e->node_tier, nodes));
+    FS_TRY(fs::dev_alloc(&e->nexit, T));
+    FS_TRY(fs::dev_alloc(&e->tile_n0, T));
+    FS_TRY(fs::dev_alloc(&e->cells_list, T * deep::kCells));
+    FS_TRY(fs::dev_alloc(&e->items, T * deep::kTiersPerTile));
+    FS_TRY(fs::dev_alloc(&e->thist, tier_words));
+    FS_TRY(fs::dev_alloc(&e->tcur, tier_words));
+    FS_TRY(fs::dev_alloc(&e->tstat, deep::kStWords));
+    FS_HIP_TRY(hipMemset(e->thist, 0, tier_words * sizeof(int32_t)));
+    FS_HIP_TRY(hipMemset(e->tstat, 0, deep::kStWords * sizeof(int32_t)));
+    if (hipHostMalloc(reinterpret_cast<void **>(&e->graph_n_h), 2 * sizeof(int32_t)) !=
+        hipSuccess) {
+      e->graph_n_h = nullptr;
+      fs::set_last_error("hipHostMalloc failed");
+      return FS_ERR_OUT_OF_MEMORY;
+    }
+  }
   FS_HIP_TRY(hipMemcpy(e->real_d, e->real.data(), count * sizeof(EnsReal), hipMemcpyHostToDevice));
+  {
+    std::vector<deep::GraphGrid> gg(count);
+    for (int r = 0; r < count; r++) {
+      const EnsReal &g = e->real[r];
+      deep::GraphGrid &d = gg[r];
+      d.off = g.off;
+      d.ftab_off = g.ftab_off;
+      d.hdr = e->rh + static_cast<size_t>(r) * kRH;   // [0] depth, [2] error bits
+      d.levels = e->lev + g.lev_off;
+      d.W = g.W;
+      d.H = g.H;
+      d.tiles_x = g.tiles_x;
+      d.tile0 = static_cast<int>(g.plan_off);
+      d.nlev = g.lev_cap;
+      d.nmode = g.nmode;
+      d.ek = g.ek;
+    }
+    FS_HIP_TRY(hipMemcpy(e->graph_grid, gg.data(), gg.size() * sizeof(deep::GraphGrid),
+                         hipMemcpyHostToDevice));
+  }
   FS_HIP_TRY(hipMemset(e->rcv, 0xFF, cells + 16));   // rings never get a receiver, RB:361-362
   FS_HIP_TRY(hipMemset(e->cnt, 0, cells * sizeof(uint32_t)));
   FS_HIP_TRY(hipMemset(e->rh, 0, static_cast<size_t>(count) * kRH * sizeof(int32_t)));
@@ -1003,6 +933,7 @@ int fs_ensemble_create(const fs_realization *reals, int count, int device, fs_en
     return FS_ERR_INTERNAL;
   }
   e->grid = per_cu * ncu;
+  e->graph_blocks = 4 * ncu;   // four k_deep_erode workgroups per CU
   const int32_t *rf = nullptr;
   FS_TRY(recv_table(e, e->sched.recv_rows, &rf));
   FS_HIP_TRY(hipDeviceSynchronize());
@@ -1078,19 +1009,7 @@ int fs_ensemble_get_accum(fs_ensemble *e, int r, double *accum, size_t count) {
   int32_t q[kRH];
   FS_TRY(download(e, q, e->rh + static_cast<size_t>(ri) * kRH, kRH));
   std::vector<uint32_t> v(count);
-  if (q[kRhPath] == 2) {
-    // the level path keeps counts in stack order: scatter them by cell
-    std::vector<int32_t> ord(count);
-    std::vector<uint32_t> cs(count);
-    FS_TRY(download(e, ord.data(), e->order + g.off, count));
-    FS_TRY(download(e, cs.data(), e->cnt_s + g.off, count));
-    for (size_t p = 0; p < count; p++) {
-      if (static_cast<uint32_t>(ord[p]) >= count) return FS_ERR_INTERNAL;
-      v[static_cast<size_t>(ord[p])] = cs[p];
-    }
-  } else {
-    FS_TRY(download(e, v.data(), e->cnt + g.off, count));
-  }
+  FS_TRY(download(e, v.data(), e->cnt + g.off, count));
   const double area = e->spec[ri].params.cell_area;
   for (size_t i = 0; i < count; i++) accum[i] = static_cast<double>(v[i]) * area;
   return FS_OK;
@@ -1110,8 +1029,13 @@ int fs_ensemble_get_levels(fs_ensemble *e, int r, int32_t *levels, size_t capaci
   const int D = q[kRhDepth];
   if (D < 0 || D + 3 > g.lev_cap) return FS_ERR_INTERNAL;
   std::vector<int32_t> lv(D + 3);
-  if (q[kRhPath] == 2) {
-    FS_TRY(download(e, lv.data(), e->lev + g.lev_off, lv.size()));
+  if (q[kRhPath] == 3) {
+    // the tile-graph step counted cells per level
+    std::vector<int32_t> c(D + 1);
+    FS_TRY(download(e, c.data(), e->lev + g.lev_off, c.size()));
+    lv[0] = 0;
+    for (int l = 0; l <= D; l++) lv[l + 1] = lv[l] + c[l];
+    lv[D + 2] = n;
   } else {
     int32_t hist[kHistLen];
     FS_TRY(download(e, hist, e->hist + static_cast<size_t>(ri) * kHistLen, kHistLen));

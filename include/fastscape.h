@@ -283,9 +283,9 @@ typedef struct fs_realization {
  *   recv_rows        rows of 256 columns per receiver work item (16..256,
  *                    a multiple of 16)
  *   tiles_per_claim  64x64 tiles per erosion work item (1..16)
- *   group_cells      cells per workgroup of a deep realization's level
- *                    path: ceil(W*H / group_cells) workgroups (at most 16)
- *                    share it; 0 = one workgroup */
+ *   group_cells      accepted (>= 0) and without effect: deep realizations
+ *                    take the tile-graph step, whose work spreads over every
+ *                    CU */
 typedef struct fs_schedule {
   int recv_rows;
   int tiles_per_claim;
@@ -305,6 +305,8 @@ int fs_ensemble_get_receivers(fs_ensemble *e, int r, int32_t *rec, size_t count)
 int fs_ensemble_get_accum(fs_ensemble *e, int r, double *accum, size_t count);
 int fs_ensemble_get_levels(fs_ensemble *e, int r, int32_t *levels, size_t capacity,
                            int32_t *nlevel);
+/* Steps realization r took on the halo tiles and off them (level_steps:
+ * drainage deeper than 16 levels, the tile-graph step). */
 int fs_ensemble_path_stats(fs_ensemble *e, int r, uint64_t *tile_steps, uint64_t *level_steps);
 int fs_ensemble_set_schedule(fs_ensemble *e, const fs_schedule *s);
 int fs_ensemble_get_schedule(fs_ensemble *e, fs_schedule *s);

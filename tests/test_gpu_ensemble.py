@@ -6,8 +6,10 @@ a single-grid model (fs_model) with its size, parameters and terrain seed:
 receivers, upstream accumulation, levels[] and elevations are compared
 exactly.  The single-grid model is itself pinned to the reference (the
 golden-vector tests), and the small cases here are also checked directly
-against the oracle.  The work schedule (receiver rows, tiles per claim,
-workgroups per deep realization) and the autotuner must not change a bit.
+against the oracle.  Realizations deeper than 16 levels take the tile-graph
+step (all of them in the same launches), the others the halo tiles.  The
+work schedule (receiver rows, tiles per claim) and the autotuner must not
+change a bit.
 """
 import sys
 
@@ -74,8 +76,8 @@ def test_mixed_ensemble_step_by_step():
             compare_all(ens, models, MIXED, "step %d" % step, stages=(step % 8 == 0 or step > 150))
         deep = [ens.path_stats(r)[1] for r in range(len(MIXED))]
         tiles = [ens.path_stats(r)[0] for r in range(len(MIXED))]
-        note("level-path steps per realization %s, tile steps %s" % (deep, tiles))
-        assert sum(deep) > 0, "no realization took the level path"
+        note("tile-graph steps per realization %s, tile steps %s" % (deep, tiles))
+        assert sum(deep) > 0, "no realization took the tile-graph step"
         assert sum(tiles) > 0, "no realization took the tile path"
         assert all(t + d == 160 for t, d in zip(tiles, deep))
         for r, (W, H, _, _) in enumerate(MIXED):
@@ -128,7 +130,7 @@ def test_schedules_give_identical_bits(sched):
 
 
 def test_schedule_matches_single_models():
-    note("grouped level path vs single-grid models")
+    note("schedule with group_cells (no effect) vs single-grid models")
     with make(SCHED_SPECS) as ens:
         ens.set_schedule(fa.schedule(recv_rows=16, tiles_per_claim=3, group_cells=4096))
         models = singles(SCHED_SPECS)
@@ -177,9 +179,9 @@ def test_set_h_resume_and_errors():
             ens.set_schedule(fa.schedule(recv_rows=20))
 
 
-def test_ensemble_level_path_matches_deep_tile_halo():
+def test_ensemble_tile_graph_matches_deep_tile_halo():
     """A realization 17-32 levels deep (48-cell conical basins, 25 levels):
-    the ensemble runs it on its level path, the single-grid model on the tile
+    the ensemble runs it on the tile graph, the single-grid model on the tile
     path with the 32-cell halo; both give the same bits at every stage."""
     note("ensemble with a 25-level realization")
     cone = cone_terrain(200, 160, 48, 0)
