@@ -148,9 +148,9 @@ struct fs_model {
   uint32_t *node_kids = nullptr, *node_parent = nullptr, *node_out = nullptr;
   int32_t *node_cell = nullptr, *nexit = nullptr;
   uint16_t *node_tier = nullptr;
-  int32_t *tile_n0 = nullptr;      // per tile: cells with a root tail
+  int32_t *tile_n0 = nullptr;      // per tile: cells with a root tail | largest hops << 16
   int32_t *thist = nullptr, *tcur = nullptr;   // per tier: work items, their cursor
-  uint16_t *cells = nullptr;       // per tile: its cells sorted by tier
+  uint32_t *cells = nullptr;       // per tile: its cells sorted by tier (list entries)
   uint64_t *items = nullptr;       // work items in tier order
   int32_t *tstat = nullptr;        // error bits, deepest tier, items, queue head
   int graph_blocks = 0;            // persistent erosion grid
@@ -501,7 +501,8 @@ int launch_erosion(fs_model *m, const std::vector<Span> &spans) {
 
 int launch_receivers(fs_model *m) {
   const unsigned gx = static_cast<unsigned>((m->W - 4 + 255) / 256);
-  const unsigned gy = static_cast<unsigned>(std::min((m->H - 4 + kRecvRows - 1) / kRecvRows, 65535));
+  const int strips = (m->H - 4 + kRecvRows - 1) / kRecvRows;
+  const unsigned gy = static_cast<unsigned>(std::min((strips + kRecvStrips - 1) / kRecvStrips, 65535));
   FS_TRY(kt_begin(m, FS_K_RECEIVERS));
   hipLaunchKernelGGL(k_receivers, dim3(gx, gy), dim3(256), 0, m->stream, m->h, m->rcv, m->W, m->H,
                      m->d_card, m->d_diag, m->hdr);
@@ -795,7 +796,9 @@ int ensure_graph(fs_model *m) {
   const size_t nodes = tiles * deep::kNodeCap;
   int ncu = 0;
   FS_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device));
-  m->graph_blocks = 4 * std::max(ncu, 1);   // four k_deep_erode workgroups per CU
+  m->graph_blocks = m->nmode == 1   ? deep::erode_grid<1>(ncu)
+                    : m->nmode == 2 ? deep::erode_grid<2>(ncu)
+                                    : deep::erode_grid<0>(ncu);
   FS_TRY(fs::dev_alloc(&m->dinfo, N));
   FS_TRY(fs::dev_alloc(&m->link, N));
   FS_TRY(fs::dev_alloc(&m->node_word, nodes));
@@ -856,8 +859,8 @@ int launch_graph_erosion(fs_model *m, int tiles) {
   FS_TRY(kt_end(m));
   FS_TRY(kt_begin(m, FS_K_GRAPH_ERODE));
   hipLaunchKernelGGL(deep::k_deep_erode<NMODE>, dim3(static_cast<unsigned>(m->graph_blocks)),
-                     dim3(deep::kEroBlock), 0, s, w, m->rcv, m->cnt, m->dinfo, m->cells, m->items,
-                     m->ftab, m->h, m->h_alt, m->tstat);
+                     dim3(deep::kEroBlock), 0, s, w, m->rcv, m->cnt, m->cells, m->items, m->ftab,
+                     m->h, m->h_alt, m->tstat);
   FS_TRY(check_launch());
   return kt_end(m);
 }

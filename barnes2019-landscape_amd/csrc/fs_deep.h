@@ -69,9 +69,10 @@ constexpr uint64_t kOneArrival = uint64_t{1} << kSumShift;
 constexpr int kTierCap = FS_MAX_LEVELS;   // a tier is at most the depth of the drainage
 // tstat words
 constexpr int kStErr = 0, kStMaxTier = 1, kStItems = 2, kStQueue = 3, kStWords = 4;
-// per-tier item counters are split over kShards words (by workgroup id, so
-// by XCD) to keep tens of thousands of atomics per step off one address
-constexpr int kShards = 8;
+// per-tier item counters are split over kShards words (by tile, so the
+// workgroups of one XCD use their own) to keep tens of thousands of atomics
+// per step off a few addresses
+constexpr int kShards = 64;
 constexpr int kBatch = 4;                 // work items a workgroup takes at once
 constexpr int32_t kErrTier = 1, kErrChain = 2;
 // the "not yet eroded" mark: a signalling NaN that no arithmetic produces
@@ -79,6 +80,15 @@ constexpr unsigned long long kUnsolved = 0x7FF4A5A5A5A5A5A5ull;
 constexpr int kSpinCap = 1 << 24;
 static_assert(kCells <= 4096, "tail and hops fit 12 bits each");
 static_assert(kTierCap < 65536, "tiers fit 16 bits");
+
+// cell list entry: the tile cell (12 bits) | the position of its receiver in
+// the cell's work item << 12 (13 bits; kNoPos: the receiver lies in another
+// tile, or the cell has a root tail)
+constexpr uint32_t kEntCell = 0xFFFu;
+constexpr int kEntShift = 12;
+constexpr uint32_t kNoPos = 0x1FFFu;
+static_assert(kCells - 1 <= static_cast<int>(kEntCell) && kCells < static_cast<int>(kNoPos),
+              "list entries");
 
 // dinfo word of a cell: tail (12 bits) | hops to the tail << 12 | exit tail
 constexpr uint32_t kInfoExit = 1u << 24;
@@ -253,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
     GraphWork w, const uint8_t *__restrict__ rcv0, uint32_t *__restrict__ cnt0,
     uint32_t *__restrict__ dinfo0, uint32_t *__restrict__ link0, uint64_t *__restrict__ node_word,
     uint32_t *__restrict__ node_kids, int32_t *__restrict__ node_cell,
-    int32_t *__restrict__ nexit, int32_t *__restrict__ tile_n0, uint16_t *__restrict__ cells,
+    int32_t *__restrict__ nexit, int32_t *__restrict__ tile_n0, uint32_t *__restrict__ cells,
     double *__restrict__ h_new0, int32_t *__restrict__ tstat) {
   __shared__ uint16_t nx[kCells];
   __shared__ uint32_t jw[kCells];    // pointer jumping: ancestor | hops to it << 16
@@ -275,13 +285,17 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
     if (threadIdx.x == 0) G.hdr[0] = 0;
     for (int q = threadIdx.x; q < G.nlev; q += kBlock) G.levels[q] = 0;
   }
+  uint8_t rr[kPer];   // receiver bytes, kept for the exits' marks
 #pragma unroll
   for (int j = 0; j < kPer; j++) {
     const int i = j * kBlock + threadIdx.x;
     const int lx = i % kSide, ly = i / kSide;
     uint16_t n = kTermNone;
-    if (lx < b.tw && ly < b.th)
-      n = tile_next(rcv[static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx], lx, ly, b);
+    rr[j] = 0xFF;
+    if (lx < b.tw && ly < b.th) {
+      rr[j] = rcv[static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx];
+      n = tile_next(rr[j], lx, ly, b);
+    }
     nx[i] = n;
     jw[i] = n < kTermNone ? (static_cast<uint32_t>(n) | (1u << 16)) : static_cast<uint32_t>(i);
     lc[i] = n != kTermNone ? 1u : 0u;
@@ -349,8 +363,7 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
       node_cell[id] = g;
       node_word[id] = lc[i];
       node_kids[id] = 0;
-      const uint8_t r = rcv[g];
-      h_new[g + dir_offset(r, W)] = __longlong_as_double(static_cast<long long>(kUnsolved));
+      h_new[g + dir_offset(rr[j], W)] = __longlong_as_double(static_cast<long long>(kUnsolved));
     }
   }
   __syncthreads();
@@ -390,12 +403,14 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
     carry += total;
   }
   __syncthreads();   // every cursor is set before any thread takes a slot
-  uint16_t *const list = cells + static_cast<size_t>(t) * kCells;
+  uint32_t *const list = cells + static_cast<size_t>(t) * kCells;
 #pragma unroll
   for (int j = 0; j < kPer; j++)
     if (rooted[j])
-      list[atomicAdd(&jw[hops[j]], 1u)] = static_cast<uint16_t>(j * kBlock + threadIdx.x);
-  if (threadIdx.x == 0) tile_n0[t] = carry;
+      list[atomicAdd(&jw[hops[j]], 1u)] =
+          static_cast<uint32_t>(j * kBlock + threadIdx.x) | kNoPos << kEntShift;
+  // n0, and the tile's largest hops (which bounds the sort keys of k_deep_counts)
+  if (threadIdx.x == 0) tile_n0[t] = carry | top << 16;
 }
 
 // ---------------------------------------------------------------------------
@@ -602,18 +617,33 @@ __global__ __launch_bounds__(kBlock) void k_deep_levels(GraphWork w,
 // ---------------------------------------------------------------------------
 // k_deep_counts: one tile per workgroup
 // ---------------------------------------------------------------------------
+constexpr int kKeyCap = 3072;    // (tier rank, hops) sort keys held apart from the elevations
+constexpr int kHashCap = 512;    // open-addressing set of the tile's exit tiers
 struct CountShared {
-  double hs[kCells];              // new elevations of the tier-0 cells
+  double hs[kCells];              // elevations, then the root-tailed cells' new ones
   uint32_t cs[kCells];
+  int keys[kKeyCap];               // sort keys, then each cell's list position
   uint16_t nx[kCells];
-  uint16_t tier_at[kCells];       // tier of each exit cell
   uint8_t done[kCells];
+  int tset[kHashCap];
+  uint16_t tier_b[4 * kSide];     // tier of the exit at each border position
   int tiers[kTiersPerTile];       // the tile's distinct tiers, then sorted
   int sorted[kTiersPerTile];
   int hist[kTiersPerTile];        // cells per tier rank, then their cursor
   int scan[kBlock / kWave + 1];
   int ntier, maxhops;
 };
+
+static_assert(sizeof(int) * kKeyCap >= sizeof(uint16_t) * kCells, "list positions fit the keys");
+
+// Border position of a border cell of the tile: rows 0 and th-1, then
+// columns 0 and tw-1 (exits lie on the border; a corner takes its row's).
+__device__ __forceinline__ int border_pos(int lx, int ly, const Box &b) {
+  if (ly == 0) return lx;
+  if (ly == b.th - 1) return kSide + lx;
+  if (lx == 0) return 2 * kSide + ly;
+  return 3 * kSide + ly;
+}
 
 __device__ __forceinline__ int tier_rank(const int *sorted, int n, int tier) {
   int lo = 0, hi = n - 1;
@@ -631,7 +661,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const uint32_t *__restrict__ dinfo0, const uint32_t *__restrict__ link0,
     const uint32_t *__restrict__ node_out, const int32_t *__restrict__ node_cell,
     const uint16_t *__restrict__ node_tier, const int32_t *__restrict__ nexit,
-    const int32_t *__restrict__ tile_n0, uint16_t *__restrict__ cells,
+    const int32_t *__restrict__ tile_n0, uint32_t *__restrict__ cells,
     int32_t *__restrict__ tcur, uint64_t *__restrict__ items, const double *__restrict__ ftab0,
     const double *__restrict__ h_old0, double *__restrict__ h_new0) {
   __shared__ CountShared sh;
@@ -650,9 +680,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
   int32_t *const hdr = G.hdr;
   const int t = wt.tg;   // per-tile arrays and node ids
   const Box b = tile_box(wt.t, G.tiles_x, W, H);
+  const int n0top = uni(tile_n0[t]);
+  const int n0 = n0top & 0xFFFF, top = n0top >> 16;
+  // the elevations now, unless the sort keys may need their array as scratch
+  const bool early_h = top + 1 <= kKeyCap;
   uint32_t info[kPer];
   if (threadIdx.x == 0) sh.ntier = sh.maxhops = 0;
   if (threadIdx.x < kTiersPerTile) sh.hist[threadIdx.x] = 0;
+  for (int q = threadIdx.x; q < kHashCap; q += kBlock) sh.tset[q] = -1;
 #pragma unroll
   for (int j = 0; j < kPer; j++) {
     const int i = j * kBlock + threadIdx.x;
@@ -664,6 +699,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
       n = tile_next(rcv[g], lx, ly, b);
       info[j] = dinfo[g];
       sh.cs[i] = cnt[g];
+      if (early_h) sh.hs[i] = h_old[g];
     }
     sh.nx[i] = n;
     sh.done[i] = 0;
@@ -674,40 +710,43 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const int id = t * kNodeCap + threadIdx.x;
     const int32_t g = node_cell[id];
     my_tier = node_tier[id];
-    sh.tier_at[(g / W - b.y0) * kSide + (g % W - b.x0)] = static_cast<uint16_t>(my_tier);
-    sh.tiers[threadIdx.x] = my_tier;
+    sh.tier_b[border_pos(g % W - b.x0, g / W - b.y0, b)] = static_cast<uint16_t>(my_tier);
   }
   __syncthreads();
-  // the tile's distinct exit tiers, sorted
+  // the tile's distinct exit tiers: the node that inserts a tier into the
+  // set lists it
   if (my_tier >= 0) {
-    bool first = true;
-    for (int q = 0; q < static_cast<int>(threadIdx.x); q++)
-      if (sh.tiers[q] == my_tier) {
-        first = false;
+    int slot = static_cast<int>((static_cast<uint32_t>(my_tier) * 2654435761u) >> 23);
+    for (;;) {   // at most kNodeCap tiers in kHashCap slots: a free slot is always found
+      const int old = atomicCAS(&sh.tset[slot], -1, my_tier);
+      if (old == -1) {
+        sh.sorted[atomicAdd(&sh.ntier, 1)] = my_tier;
         break;
       }
-    if (first) sh.sorted[atomicAdd(&sh.ntier, 1)] = my_tier;
+      if (old == my_tier) break;
+      slot = (slot + 1) & (kHashCap - 1);
+    }
   }
   // inflows: a ring cell that drains into the tile is an exit of its own
   // tile; its count is added along the in-tile chain from the cell it
   // drains into
   {
     const int q = threadIdx.x;
-    const int top = b.tw + 2, nring = 2 * top + 2 * b.th;
+    const int ring_top = b.tw + 2, nring = 2 * ring_top + 2 * b.th;
     if (q < nring) {
       int gx, gy;
-      if (q < top) {
+      if (q < ring_top) {
         gx = b.x0 - 1 + q;
         gy = b.y0 - 1;
-      } else if (q < 2 * top) {
-        gx = b.x0 - 1 + (q - top);
+      } else if (q < 2 * ring_top) {
+        gx = b.x0 - 1 + (q - ring_top);
         gy = b.y0 + b.th;
-      } else if (q < 2 * top + b.th) {
+      } else if (q < 2 * ring_top + b.th) {
         gx = b.x0 - 1;
-        gy = b.y0 + (q - 2 * top);
+        gy = b.y0 + (q - 2 * ring_top);
       } else {
         gx = b.x0 + b.tw;
-        gy = b.y0 + (q - 2 * top - b.th);
+        gy = b.y0 + (q - 2 * ring_top - b.th);
       }
       if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
         const size_t g = static_cast<size_t>(gy) * W + gx;
@@ -735,7 +774,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const int v = sh.sorted[threadIdx.x];
     int rank = 0;
     for (int q = 0; q < ntier; q++) rank += sh.sorted[q] < v;
-    sh.tiers[rank] = v;   // tiers[] no longer needed as node tiers
+    sh.tiers[rank] = v;
   }
   __syncthreads();
   // final counts; each exit-tailed cell's tier rank
@@ -747,35 +786,43 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     if (sh.nx[i] == kTermNone) continue;
     cnt[static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide] = sh.cs[i];
     if (!(info[j] & kInfoExit)) continue;
-    rank[j] = tier_rank(sh.tiers, ntier, sh.tier_at[info_tail(info[j])]);
+    const int tl = info_tail(info[j]);
+    rank[j] = tier_rank(sh.tiers, ntier, sh.tier_b[border_pos(tl % kSide, tl / kSide, b)]);
     atomicAdd(&sh.hist[rank[j]], 1);
   }
   __syncthreads();
-  const int n0 = tile_n0[t];
   int mh = 0;
 #pragma unroll
   for (int j = 0; j < kPer; j++)
     if (rank[j] >= 0) mh = max(mh, info_hops(info[j]));
   const int span = block_max(mh, &sh.maxhops) + 1;   // hops values per tier
+  // the work items: tier ranks in order, each its cells' count; the slot
+  // is taken now and written after the sort, which hides the atomic's
+  // round trip
+  int item_pos = -1;
+  uint64_t item_word = 0;
   {
-    // the work items: tier ranks in order, each its cells' count
     const int v = static_cast<int>(threadIdx.x) < ntier ? sh.hist[threadIdx.x] : 0;
     int total;
     const int ex = block_scan(v, sh.scan, &total);
     if (static_cast<int>(threadIdx.x) < ntier) {
       const int tier = sh.tiers[threadIdx.x];
-      const int pos = atomicAdd(&tcur[tier * kShards + (t & (kShards - 1))], 1);
-      items[pos] = item_pack(wi, n0 + ex, v);
+      item_pos = atomicAdd(&tcur[tier * kShards + (t & (kShards - 1))], 1);
+      item_word = item_pack(wi, n0 + ex, v);
       sh.hist[threadIdx.x] = n0 + ex;   // first slot of the rank
     }
   }
   // each rank's cells sorted by hops (a work item solves its cells in list
-  // order): one counting sort over (rank, hops) keys when they fit the
-  // elevation array, which is not loaded yet; else one per rank
-  int *const keyc = reinterpret_cast<int *>(sh.hs);
-  constexpr int kKeys = static_cast<int>(sizeof(sh.hs) / sizeof(int));
-  uint16_t *const list = cells + static_cast<size_t>(t) * kCells;
-  const int passes = ntier * span <= kKeys ? 1 : ntier;
+  // order): one counting sort over (rank, hops) keys when they fit the key
+  // array, else one per rank; hops beyond the key array (only when the
+  // elevations were not loaded yet) count in the elevation array
+  const bool big = span > kKeyCap;
+  int *const keyc = big ? reinterpret_cast<int *>(sh.hs) : sh.keys;
+  uint32_t *const list = cells + static_cast<size_t>(t) * kCells;
+  const int passes = ntier * span <= kKeyCap ? 1 : ntier;
+  int lp[kPer];   // each exit-tailed cell's position in the list
+#pragma unroll
+  for (int j = 0; j < kPer; j++) lp[j] = -1;
   for (int pass = 0; pass < passes; pass++) {
     const int nkeys = passes == 1 ? ntier * span : span;
     for (int q = threadIdx.x; q < nkeys; q += kBlock) keyc[q] = 0;
@@ -789,8 +836,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
       atomicAdd(&keyc[key[j]], 1);
     }
     __syncthreads();
-    int carry = passes == 1 ? 0 : sh.hist[pass];
-    if (passes == 1) carry = n0;
+    int carry = passes == 1 ? n0 : sh.hist[pass];
     for (int base = 0; base < nkeys; base += kBlock) {
       const int q = base + threadIdx.x;
       const int v = q < nkeys ? keyc[q] : 0;
@@ -802,25 +848,45 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     __syncthreads();   // every cursor is set before any thread takes a slot
 #pragma unroll
     for (int j = 0; j < kPer; j++)
-      if (key[j] >= 0)
-        list[atomicAdd(&keyc[key[j]], 1)] = static_cast<uint16_t>(j * kBlock + threadIdx.x);
+      if (key[j] >= 0) lp[j] = atomicAdd(&keyc[key[j]], 1);
     __syncthreads();
   }
-  // the tile's elevations, now that the sort's scratch is free
+  // the list entries: each cell with the position of its in-tile receiver
+  // (same tail, so the same work item, one hop earlier) relative to the item
+  {
+    uint16_t *const lpos = reinterpret_cast<uint16_t *>(sh.keys);   // free after the sort
 #pragma unroll
-  for (int j = 0; j < kPer; j++) {
-    const int i = j * kBlock + threadIdx.x;
-    if (sh.nx[i] != kTermNone)
-      sh.hs[i] = h_old[static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide];
+    for (int j = 0; j < kPer; j++)
+      if (lp[j] >= 0) lpos[j * kBlock + threadIdx.x] = static_cast<uint16_t>(lp[j]);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      if (lp[j] < 0) continue;
+      const int i = j * kBlock + threadIdx.x;
+      const int ri = sh.nx[i];
+      const uint32_t rp = ri < kTermNone
+                              ? static_cast<uint32_t>(lpos[ri] - sh.hist[rank[j]])
+                              : kNoPos;
+      list[lp[j]] = static_cast<uint32_t>(i) | rp << kEntShift;
+    }
   }
-  __syncthreads();
-  // tier 0: the cells with a root tail, from the list k_deep_local sorted
+  if (item_pos >= 0) items[item_pos] = item_word;
+  if (!early_h) {   // the elevations, now that the sort's scratch is free
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const int i = j * kBlock + threadIdx.x;
+      if (sh.nx[i] != kTermNone)
+        sh.hs[i] = h_old[static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide];
+    }
+    __syncthreads();
+  }
+  // the cells with a root tail, from the list k_deep_local sorted
   // by hops; each wave takes 64-cell chunks in order and a lane solves its
   // cell once its receiver (an earlier cell of the list) is done
   for (int ch = wave; ch * kWave < n0; ch += kBlock / kWave) {
     const int p = ch * kWave + lane;
     if (p >= n0) continue;
-    const int i = list[p];
+    const int i = static_cast<int>(list[p] & kEntCell);
     const int ri = sh.nx[i];
     const bool root = ri == kTermRoot;
     double fact = 0.0;
@@ -865,21 +931,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
 // ---------------------------------------------------------------------------
 // k_deep_erode: persistent; one work item at a time per workgroup
 // ---------------------------------------------------------------------------
-constexpr int kEroBlock = 256;   // four workgroups per CU: items are small and latency-bound
+constexpr int kEroBlock = 256;   // items are small and latency-bound: many workgroups per CU
+constexpr int kWin = 1024;       // cells of an item held in LDS at once
 struct ErodeShared {
-  double hs[kCells];        // new elevation of each tile cell of the item
-  uint8_t done[kCells];
+  double hs[kWin];          // new elevation of each cell of the window, by position
+  uint8_t done[kWin];
   uint64_t item;
 };
 
-// Work items of tiers 1 and up; an item's cells come sorted by hops, so a
-// cell's receiver in the tile (one hop nearer the exit) is always earlier.
+// Work items (the exit-tailed cells); an item's cells come sorted by hops, so
+// a cell's receiver in the tile (one hop nearer the exit) is always earlier.
+// The item is solved in windows of kWin cells: a receiver in the window is
+// read from LDS, one in an earlier window of the item from the new elevation
+// field (final: stored before the window's barrier), one in another tile from
+// the field once its mark is replaced.
 template <int NMODE>
 __global__ __launch_bounds__(kEroBlock) void k_deep_erode(
     GraphWork w, const uint8_t *__restrict__ rcv0, const uint32_t *__restrict__ cnt0,
-    const uint32_t *__restrict__ dinfo0, const uint16_t *__restrict__ cells,
-    const uint64_t *__restrict__ items, const double *__restrict__ ftab0,
-    const double *__restrict__ h_old0, double *h_new0, int32_t *tstat) {
+    const uint32_t *__restrict__ cells, const uint64_t *__restrict__ items,
+    const double *__restrict__ ftab0, const double *__restrict__ h_old0, double *h_new0,
+    int32_t *tstat) {
   __shared__ ErodeShared sh;
   const int nitems = tstat[kStItems];
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -902,70 +973,85 @@ __global__ __launch_bounds__(kEroBlock) void k_deep_erode(
     const int n = static_cast<int>(item >> 40);
     const int W = G.W;
     const uint8_t *const rcv = rcv0 + G.off;
-    const uint32_t *const cnt = cnt0 + G.off, *const dinfo = dinfo0 + G.off;
+    const uint32_t *const cnt = cnt0 + G.off;
     const double *const ftab = ftab0 + G.ftab_off, *const h_old = h_old0 + G.off;
     double *const h_new = h_new0 + G.off;
     const ErodeConst k = G.ek;
     int32_t *const hdr = G.hdr;
     const Box b = tile_box(wt.t, G.tiles_x, W, G.H);
-    const uint16_t *const tl = cells + static_cast<size_t>(wt.tg) * kCells + first;
-    for (int p = threadIdx.x; p < n; p += kEroBlock) sh.done[tl[p]] = 0;
-    __syncthreads();
-    // wavefront: each wave takes 64-cell chunks in order; a lane solves its
-    // cell as soon as its receiver is done (an earlier cell of the list, or
-    // for an exit a cell of a lower tier in another tile).  The wait, the
-    // solve and the publication share one loop, so a lane whose receiver is
-    // on its own wave gets it on a later pass.
-    for (int ch = wave; ch * kWave < n; ch += kEroBlock / kWave) {
-      const int p = ch * kWave + lane;
-      if (p >= n) continue;
-      const int i = tl[p];
-      const int lx = i % kSide, ly = i / kSide;
-      const size_t g = static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx;
-      const uint8_t r = rcv[g];
-      const double h0 = h_old[g];
-      const bool ext = info_hops(dinfo[g]) == 0;   // an exit: receiver in another tile
-      const double fact = ftab[2 * static_cast<size_t>(cnt[g]) + (r & 1)];
-      const int ri = i + dir_dy(r) * kSide + dir_dx(r);
-      const unsigned long long *rp = reinterpret_cast<const unsigned long long *>(
-          h_new + static_cast<long long>(g) + dir_offset(r, W));
-      int spins = 0;
-      for (;;) {
-        bool ready;
-        double hr = 0.0;
-        if (ext) {
-          const unsigned long long v =
-              __hip_atomic_load(rp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          ready = v != kUnsolved;
-          hr = __longlong_as_double(static_cast<long long>(v));
-        } else {
-          ready = __hip_atomic_load(&sh.done[ri], __ATOMIC_ACQUIRE,
-                                    __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
-          if (ready) hr = sh.hs[ri];
+    const uint32_t *const tl = cells + static_cast<size_t>(wt.tg) * kCells + first;
+    for (int w0 = 0; w0 < n; w0 += kWin) {
+      const int wn = min(kWin, n - w0);
+      for (int q = threadIdx.x; q < wn; q += kEroBlock) sh.done[q] = 0;
+      __syncthreads();
+      // wavefront: each wave takes 64-cell chunks in order; a lane solves its
+      // cell as soon as its receiver is done.  The wait, the solve and the
+      // publication share one loop, so a lane whose receiver is on its own
+      // wave gets it on a later pass.
+      for (int ch = wave; ch * kWave < wn; ch += kEroBlock / kWave) {
+        const int q = ch * kWave + lane;
+        if (q >= wn) continue;
+        const uint32_t e = tl[w0 + q];
+        const int i = static_cast<int>(e & kEntCell);
+        const int rp = static_cast<int>(e >> kEntShift);
+        const bool local = rp != static_cast<int>(kNoPos) && rp >= w0;
+        const int lx = i % kSide, ly = i / kSide;
+        const size_t g = static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx;
+        const uint8_t r = rcv[g];
+        const double h0 = h_old[g];
+        const double fact = ftab[2 * static_cast<size_t>(cnt[g]) + (r & 1)];
+        const unsigned long long *rp_g = reinterpret_cast<const unsigned long long *>(
+            h_new + static_cast<long long>(g) + dir_offset(r, W));
+        int spins = 0;
+        for (;;) {
+          bool ready;
+          double hr = 0.0;
+          if (local) {
+            ready = __hip_atomic_load(&sh.done[rp - w0], __ATOMIC_ACQUIRE,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+            if (ready) hr = sh.hs[rp - w0];
+          } else {
+            const unsigned long long v =
+                __hip_atomic_load(rp_g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ready = v != kUnsolved;
+            hr = __longlong_as_double(static_cast<long long>(v));
+          }
+          const bool give_up = !ready && ++spins > kSpinCap;
+          if (ready || give_up) {
+            double hn = h0;
+            if (give_up) atomicOr(&hdr[2], kErrLookback);
+            else hn = solve<NMODE>(G.nmode, h0 + k.ud, hr, fact, k, hdr);
+            sh.hs[q] = hn;
+            __hip_atomic_store(&sh.done[q], static_cast<uint8_t>(1), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+            // every new elevation goes past this XCD's cache: border cells
+            // may be the receivers of other tiles' exits, and a cache line
+            // holds cells of two tiles wherever a row does not start on a
+            // line, so a plain store beside such a receiver would leave this
+            // XCD's L2 with a copy of the line that its polls could read
+            __hip_atomic_store(reinterpret_cast<unsigned long long *>(h_new + g),
+                               static_cast<unsigned long long>(__double_as_longlong(hn)),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
         }
-        const bool give_up = !ready && ++spins > kSpinCap;
-        if (ready || give_up) {
-          double hn = h0;
-          if (give_up) atomicOr(&hdr[2], kErrLookback);
-          else hn = solve<NMODE>(G.nmode, h0 + k.ud, hr, fact, k, hdr);
-          sh.hs[i] = hn;
-          __hip_atomic_store(&sh.done[i], static_cast<uint8_t>(1), __ATOMIC_RELEASE,
-                             __HIP_MEMORY_SCOPE_WORKGROUP);
-          // every new elevation goes past this XCD's cache: border cells
-          // may be the receivers of other tiles' exits, and a cache line
-          // holds cells of two tiles wherever a row does not start on a
-          // line, so a plain store beside such a receiver would leave this
-          // XCD's L2 with a copy of the line that its polls could read
-          __hip_atomic_store(reinterpret_cast<unsigned long long *>(h_new + g),
-                             static_cast<unsigned long long>(__double_as_longlong(hn)),
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
+}
+
+// The persistent erosion grid: as many workgroups as fit on the device at
+// once (host side).
+template <int NMODE>
+int erode_grid(int ncu) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_deep_erode<NMODE>, kEroBlock, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return per_cu * (ncu < 1 ? 1 : ncu);
 }
 
 __global__ void k_deep_clear(int32_t *tstat) {

@@ -182,34 +182,98 @@ __global__ void k_fill_counts(int32_t *hdr, int n_counts) {
   if (i <= kJumpPasses) hdr[kHdrJump + i] = 0;
 }
 
-// ComputeReceivers (RB:160-182).  Each thread owns one column of a strip of
-// kRecvRows interior rows and slides a 3x3 window down it, so every
-// elevation is fetched from HBM about once (neighbouring columns come from
-// L1).  fs::receiver_fast picks the reference's direction -- steepest
+// ComputeReceivers (RB:160-182).  Each thread owns one column of a run of
+// interior rows (whole strips of kRecvRows) and slides a 3x3 window down it,
+// so every elevation is fetched from HBM about once (neighbouring columns
+// come from L1, and a run re-reads only its two halo rows).
+// fs::receiver_try picks the reference's direction -- steepest
 // (hc - h[n]) / dr[n] over n = 0..7 with a strict '>' from 0, ties to the
-// lowest n, flat cells and pits NO_FLOW -- usually without dividing, else
-// with two IEEE divisions instead of eight (see fs_receiver.h).
+// lowest n, flat cells and pits NO_FLOW -- without dividing; the rare cells it
+// leaves undecided (near-ties, extreme magnitudes) send the column's run
+// through fs::receiver_fast again, which divides where it must, so the row
+// loop has no branch.  Rows are read and written through buffer resources
+// built per row in scalar registers: the lane's column is a constant offset,
+// so the loop spends no vector instructions on addresses.
 constexpr int kRecvRows = 16;
-__device__ __forceinline__ void receivers_strip(const double *__restrict__ h,
-                                                uint8_t *__restrict__ rcv, int W, int H, int x,
-                                                int strip, double d_card, double d_diag) {
-  const int y0 = strip * kRecvRows + 2;
-  const int y1 = min(y0 + kRecvRows, H - 2);
+constexpr int kRecvStrips = 2;                  // strips per thread in k_receivers
+constexpr int kRsrcWord3 = 0x00020000;          // raw (unformatted) buffer, gfx9
+using v4u = unsigned __attribute__((ext_vector_type(4)));
+using v2u = unsigned __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void *row, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(row), 0, static_cast<int>(bytes),
+                                           kRsrcWord3);
+}
+
+// columns x-1, x, x+1 of a row; xo = 8*(x-1) (a row of W doubles is below
+// 2^32 bytes: W*H < 2^31 with H >= 5)
+__device__ __forceinline__ void load_cols(__amdgpu_buffer_rsrc_t r, uint32_t xo, double &l,
+                                          double &c, double &rr) {
+  const v4u a = __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(xo), 0, 0);
+  const v2u b = __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(xo) + 16, 0, 0);
+  l = __builtin_bit_cast(double, v2u{a.x, a.y});
+  c = __builtin_bit_cast(double, v2u{a.z, a.w});
+  rr = __builtin_bit_cast(double, b);
+}
+
+// Rows y0 .. y0+n-1 (all interior: 2 <= y0, y0+n <= H-2) of column x.  The
+// window holds rows y-1, y, y+1 and the next row's load is in flight while
+// row y is decided (row y+2 <= H-1 always exists).
+__device__ __forceinline__ void receivers_rows(const double *__restrict__ h,
+                                               uint8_t *__restrict__ rcv, int W, int x, int y0,
+                                               int n, double d_card, double d_diag) {
   const bool div_ok = fs::divisors_in_range(d_card, d_diag);
-  const double *p = h + static_cast<size_t>(y0 - 1) * W + x;
-  double a0 = p[-1], a1 = p[0], a2 = p[1];        // row y-1
-  p += W;
-  double b0 = p[-1], b1 = p[0], b2 = p[1];        // row y
-  for (int y = y0; y < y1; y++) {
-    p += W;
-    const double c0 = p[-1], c1 = p[0], c2 = p[1];  // row y+1
+  const uint32_t rowb = static_cast<uint32_t>(W) * 8u;
+  const uint32_t xo = static_cast<uint32_t>(x - 1) * 8u;
+  const double *row = h + static_cast<size_t>(y0 - 1) * W;   // row y-1 of the first cell
+  uint8_t *out = rcv + static_cast<size_t>(y0) * W;
+  double a0, a1, a2, b0, b1, b2, c0, c1, c2;
+  load_cols(row_rsrc(row, rowb), xo, a0, a1, a2);
+  load_cols(row_rsrc(row + W, rowb), xo, b0, b1, b2);
+  load_cols(row_rsrc(row + 2 * W, rowb), xo, c0, c1, c2);
+  uint32_t undecided = 0;
+  auto one_row = [&]() {
+    double d0, d1, d2;
+    load_cols(row_rsrc(row + 3 * W, rowb), xo, d0, d1, d2);
     // neighbours in direction order W, NW, N, NE, E, SE, S, SW (RB:79-82)
     const double nb[8] = {b0, a0, a1, a2, b2, c2, c1, c0};
-    rcv[static_cast<size_t>(y) * W + x] =
-        static_cast<uint8_t>(fs::receiver_fast(b1, nb, d_card, d_diag, div_ok));
+    bool ok;
+    const int r = fs::receiver_try(b1, nb, d_card, d_diag, div_ok, &ok);
+    __builtin_amdgcn_raw_buffer_store_b8(static_cast<uint8_t>(r), row_rsrc(out, W), x, 0, 0);
+    undecided |= ok ? 0u : 1u;
+    // settle the flag here: left symbolic, every row's comparison masks stay
+    // live in scalar registers until after the loop
+    asm volatile("" : "+v"(undecided));
+    row += W;
+    out += W;
     a0 = b0; a1 = b1; a2 = b2;
     b0 = c0; b1 = c1; b2 = c2;
+    c0 = d0; c1 = d1; c2 = d2;
+  };
+  int k = 0;
+  for (; k + kRecvRows <= n; k += kRecvRows) {
+#pragma unroll
+    for (int u = 0; u < kRecvRows; u++) one_row();
   }
+  for (; k < n; k++) one_row();
+  if (undecided) {   // rare: the column's rows again, with the divisions where needed
+    for (k = 0; k < n; k++) {
+      const double *up = h + static_cast<size_t>(y0 + k - 1) * W + x;
+      const double *mid = up + W, *dn = mid + W;
+      const double nb[8] = {mid[-1], up[-1], up[0], up[1], mid[1], dn[1], dn[0], dn[-1]};
+      rcv[static_cast<size_t>(y0 + k) * W + x] =
+          static_cast<uint8_t>(fs::receiver_fast(mid[0], nb, d_card, d_diag, div_ok));
+    }
+  }
+}
+
+// Strips s0 .. s1-1 of kRecvRows interior rows each, as one run of rows.
+__device__ __forceinline__ void receivers_strips(const double *__restrict__ h,
+                                                 uint8_t *__restrict__ rcv, int W, int H, int x,
+                                                 int s0, int s1, double d_card, double d_diag) {
+  const int y0 = s0 * kRecvRows + 2;
+  const int n = min(s1 * kRecvRows + 2, H - 2) - y0;
+  if (n > 0) receivers_rows(h, rcv, W, x, y0, n, d_card, d_diag);
 }
 
 // Also clears the header words the tile path's accumulation reports into
@@ -226,8 +290,8 @@ __global__ __launch_bounds__(256) void k_receivers(const double *__restrict__ h,
   const int x = blockIdx.x * 256 + threadIdx.x + 2;
   if (x >= W - 2) return;
   const int nstrips = (H - 4 + kRecvRows - 1) / kRecvRows;
-  for (int strip = blockIdx.y; strip < nstrips; strip += gridDim.y)
-    receivers_strip(h, rcv, W, H, x, strip, d_card, d_diag);
+  for (int s = blockIdx.y * kRecvStrips; s < nstrips; s += gridDim.y * kRecvStrips)
+    receivers_strips(h, rcv, W, H, x, s, min(s + kRecvStrips, nstrips), d_card, d_diag);
 }
 
 // Donor bitmask and level of every cell.  Rows are strided over the grid so

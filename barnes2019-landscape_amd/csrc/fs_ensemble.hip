@@ -132,23 +132,6 @@ __device__ __forceinline__ void counter_barrier(int32_t *bar, uint32_t &target, 
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-// Reserve k slots per lane from *ctr with one atomic per wave; returns the
-// lane's first slot.  Every lane of the wave must call it.
-__device__ __forceinline__ int wave_reserve(int32_t *ctr, int k) {
-  const int lane = threadIdx.x & (kWave - 1);
-  int x = k;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const int y = __shfl_up(x, d);
-    if (lane >= d) x += y;
-  }
-  const int total = __shfl(x, kWave - 1);
-  int base = 0;
-  if (lane == kWave - 1 && total > 0) base = atomicAdd(ctr, total);
-  base = __shfl(base, kWave - 1);
-  return base + x - k;
-}
-
 // Last index i in [0, n) with off[i] <= v (off ascending, off[0] <= v).
 __device__ __forceinline__ int find_entry(const int32_t *off, int n, int v) {
   int lo = 0, hi = n - 1;
@@ -246,8 +229,7 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_step(EnsArgs a) {
       if (x >= g.W - 2) continue;
       const int nstrips = (g.H - 4 + kRecvRows - 1) / kRecvRows;
       const int s0 = (local / g.recv_xb) * spi, s1 = min(nstrips, s0 + spi);
-      for (int s = s0; s < s1; s++)
-        receivers_strip(a.h_old + g.off, a.rcv + g.off, g.W, g.H, x, s, g.d_card, g.d_diag);
+      receivers_strips(a.h_old + g.off, a.rcv + g.off, g.W, g.H, x, s0, s1, g.d_card, g.d_diag);
     }
   }
   counter_barrier(&a.ctl[kCtlBarrier], bar, gridDim.x, a.ctl);
@@ -376,7 +358,8 @@ struct fs_ensemble {
   uint32_t *node_kids = nullptr, *node_parent = nullptr, *node_out = nullptr;
   uint32_t *node_up = nullptr, *node_level = nullptr;
   int32_t *node_cell = nullptr, *nexit = nullptr, *tile_n0 = nullptr;
-  uint16_t *node_tier = nullptr, *cells_list = nullptr;
+  uint16_t *node_tier = nullptr;
+  uint32_t *cells_list = nullptr;
   int32_t *thist = nullptr, *tcur = nullptr, *tstat = nullptr;
   int graph_blocks = 0;
   uint64_t graph_tiles = 0;                   // tile-graph tiles stepped so far
@@ -507,8 +490,8 @@ int graph_launches(fs_ensemble *e, const deep::GraphWork &w, int T) {
                      e->cnt, e->dinfo, e->link, e->node_out, e->node_cell, e->node_tier, e->nexit,
                      e->tile_n0, e->cells_list, e->tcur, e->items, e->ftab, h_old, h_new);
   hipLaunchKernelGGL(deep::k_deep_erode<NM>, dim3(static_cast<unsigned>(e->graph_blocks)),
-                     dim3(deep::kEroBlock), 0, s, w, e->rcv, e->cnt, e->dinfo, e->cells_list,
-                     e->items, e->ftab, h_old, h_new, e->tstat);
+                     dim3(deep::kEroBlock), 0, s, w, e->rcv, e->cnt, e->cells_list, e->items,
+                     e->ftab, h_old, h_new, e->tstat);
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) {
     fs::set_last_error(std::string("tile-graph launch: ") + hipGetErrorString(err));
@@ -933,7 +916,9 @@ int fs_ensemble_create(const fs_realization *reals, int count, int device, fs_en
     return FS_ERR_INTERNAL;
   }
   e->grid = per_cu * ncu;
-  e->graph_blocks = 4 * ncu;   // four k_deep_erode workgroups per CU
+  e->graph_blocks = e->nm_all == 1   ? deep::erode_grid<1>(ncu)
+                    : e->nm_all == 2 ? deep::erode_grid<2>(ncu)
+                                     : deep::erode_grid<-1>(ncu);
   const int32_t *rf = nullptr;
   FS_TRY(recv_table(e, e->sched.recv_rows, &rf));
   FS_HIP_TRY(hipDeviceSynchronize());

@@ -84,24 +84,28 @@ __host__ __device__ inline bool divisors_in_range(double dc, double dd) {
 }
 
 // The reference's direction, decided without division whenever every
-// quotient involved is a normal number and no two slopes are near-tied.
+// quotient involved is a normal number and no two slopes are near-tied;
+// *ok is false when that decision does not hold and the result is to be
+// replaced by receiver_two_div's.
 //
 // Within a group, RN(d/dr) is monotone in d, so the group's steepest slope
 // is RN(A/dr) for its largest drop A, and a smaller drop can round to the
-// same quotient only when it lies within 2^-52*A of A: drops below the
-// window w = A - A*2^-50 never tie with A.  If the lowest-index drop in
-// [w, A] is A itself, that index is the group's answer; otherwise a
-// near-tie needs the divisions.  Between the groups, for drops Ac, Ad and
-// divisors dc, dd in range, P = RN(Ac*dd) and Q = RN(Ad*dc) carry relative
-// errors of at most 2^-53 each, so P > RN(Q*(1 + 2^-48)) gives
-// Ac/dc > (Ad/dd)*(1 + 2^-49), hence RN(Ac/dc) > RN(Ad/dd) strictly: the
-// cardinal group wins without a tie (and symmetrically).  Anything closer,
-// or out of range, goes to receiver_two_div.
+// same quotient only when it lies within dr*ulp(q) <= 2^-52*A*(1 + 2^-53) of
+// A.  The window bound w = RN(A*(1 - 2^-50)) lies below A - 2^-51*A (the
+// product is exact up to half an ulp of A), so drops below w never tie with
+// A.  If the lowest-index drop in [w, A] is A itself, that index is the
+// group's answer; otherwise a near-tie needs the divisions.  Between the
+// groups, for drops Ac, Ad and divisors dc, dd in range, P = RN(Ac*dd) and
+// Q = RN(Ad*dc) carry relative errors of at most 2^-53 each, so
+// P > RN(Q*(1 + 2^-48)) gives Ac/dc > (Ad/dd)*(1 + 2^-49), hence
+// RN(Ac/dc) > RN(Ad/dd) strictly: the cardinal group wins without a tie (and
+// symmetrically).  Anything closer, or out of range, is left to
+// receiver_two_div.
 //
-// Written as straight-line selects: the comparisons combine as lane masks
-// and only the rare fallback branches.
-__host__ __device__ inline int receiver_fast(double hc, const double nb[8], double dc, double dd,
-                                             bool div_ok) {
+// Written as straight-line selects with no branch, so callers can decide a
+// run of cells and fix up the rare undecided ones afterwards.
+__host__ __device__ inline int receiver_try(double hc, const double nb[8], double dc, double dd,
+                                            bool div_ok, bool *ok) {
   double d[8];
   for (int n = 0; n < 8; n++) d[n] = hc - nb[n];
   // Group maxima.  fmax ignores NaN drops, as the reference's strict '>'
@@ -110,7 +114,7 @@ __host__ __device__ inline int receiver_fast(double hc, const double nb[8], doub
   const double Ac = fmax(fmax(d[0], d[2]), fmax(d[4], d[6]));
   const double Ad = fmax(fmax(d[1], d[3]), fmax(d[5], d[7]));
   const bool pc = Ac > 0.0, pd = Ad > 0.0;
-  const double wc = Ac - Ac * 0x1p-50, wd = Ad - Ad * 0x1p-50;
+  const double wc = Ac * (1.0 - 0x1p-50), wd = Ad * (1.0 - 0x1p-50);
   // lowest in-window index of each group, and whether that drop is A
   const bool c0 = d[0] >= wc, c2 = d[2] >= wc, c4 = d[4] >= wc;
   const bool d1 = d[1] >= wd, d3 = d[3] >= wd, d5 = d[5] >= wd;
@@ -126,12 +130,16 @@ __host__ __device__ inline int receiver_fast(double hc, const double nb[8], doub
   const bool rd = (Ad >= 0x1p-900) & (Ad <= 0x1p900);
   const double P = Ac * dd, Q = Ad * dc;
   const bool cw = P > Q * (1.0 + 0x1p-48), dw = Q > P * (1.0 + 0x1p-48);
-  const bool fast = div_ok & (!pc | (rc & fc)) & (!pd | (rd & fd)) & (!(pc & pd) | cw | dw);
-  if (fast) {
-    if (!pc & !pd) return kNoReceiver;
-    return (pc & (!pd | cw)) ? ic : id;
-  }
-  return receiver_two_div(hc, nb, dc, dd);
+  *ok = div_ok & (!pc | (rc & fc)) & (!pd | (rd & fd)) & (!(pc & pd) | cw | dw);
+  const int best = (pc & (!pd | cw)) ? ic : id;
+  return (!pc & !pd) ? kNoReceiver : best;
+}
+
+__host__ __device__ inline int receiver_fast(double hc, const double nb[8], double dc, double dd,
+                                             bool div_ok) {
+  bool ok;
+  const int r = receiver_try(hc, nb, dc, dd, div_ok, &ok);
+  return ok ? r : receiver_two_div(hc, nb, dc, dd);
 }
 
 __host__ __device__ inline int receiver_fast(double hc, const double nb[8], double dc, double dd) {

@@ -642,7 +642,7 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
       const int sl = q * kEroBlock + threadIdx.x;
       rv[q] = sl < kTileCells + nh ? plan->rslot[sl] : kPlanRoot;
     }
-    const bool halo = threadIdx.x < nh;
+    const bool halo = static_cast<int>(threadIdx.x) < nh;
     const int hr = halo ? plan->hreg[threadIdx.x] : 0;
     const uint8_t hl = halo ? plan->hlvl[threadIdx.x] : 0;
     const size_t hg = static_cast<size_t>(ry0 + hr / kReg) * W + (rx0 + hr % kReg);
