@@ -256,6 +256,40 @@ __device__ __forceinline__ int block_scan(int v, int *sh, int *total) {
   return r;
 }
 
+// Border position of a border cell of the tile: rows 0 and th-1, then
+// columns 0 and tw-1 (exits lie on the border; a corner takes its row's).
+__device__ __forceinline__ int border_pos(int lx, int ly, const Box &b) {
+  if (ly == 0) return lx;
+  if (ly == b.th - 1) return kSide + lx;
+  if (lx == 0) return 2 * kSide + ly;
+  return 3 * kSide + ly;
+}
+
+// The cell at border position k, or -1 (a position beyond a partial tile,
+// or a corner counted with its row).
+__device__ __forceinline__ int border_cell(int k, const Box &b) {
+  int lx, ly;
+  bool ok;
+  if (k < kSide) {
+    lx = k;
+    ly = 0;
+    ok = lx < b.tw;
+  } else if (k < 2 * kSide) {
+    lx = k - kSide;
+    ly = b.th - 1;
+    ok = lx < b.tw && b.th > 1;
+  } else if (k < 3 * kSide) {
+    lx = 0;
+    ly = k - 2 * kSide;
+    ok = ly >= 1 && ly < b.th - 1;
+  } else {
+    lx = b.tw - 1;
+    ly = k - 3 * kSide;
+    ok = ly >= 1 && ly < b.th - 1 && b.tw > 1;
+  }
+  return ok ? ly * kSide + lx : -1;
+}
+
 // ---------------------------------------------------------------------------
 // k_deep_local: one tile per workgroup
 // ---------------------------------------------------------------------------
@@ -268,8 +302,9 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
   __shared__ uint16_t nx[kCells];
   __shared__ uint32_t jw[kCells];    // pointer jumping: ancestor | hops to it << 16
   __shared__ uint32_t lc[kCells];
-  __shared__ uint16_t nid[kCells];   // node number of each exit within the tile
-  __shared__ int nnode, maxhops, scan[kBlock / kWave + 1];
+  __shared__ uint16_t slot_node[4 * kSide];   // node number of the exit at each border position
+  __shared__ int wave_nodes[4 * kSide / kWave];
+  __shared__ int maxhops, scan[kBlock / kWave + 1];
   const int wi = blockIdx.x;   // one tile per workgroup; the grid covers the list exactly
   if (wi >= w.size[1]) return;
   const WorkTile wt = work_tile(w, wi);
@@ -280,7 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
   double *const h_new = h_new0 + G.off;
   const int t = wt.tg;   // per-tile arrays and node ids
   const Box b = tile_box(wt.t, G.tiles_x, W, H);
-  if (threadIdx.x == 0) nnode = maxhops = 0;
+  if (threadIdx.x == 0) maxhops = 0;
   if (wt.t == 0 && G.levels) {   // the grid's level counts start from zero
     if (threadIdx.x == 0) G.hdr[0] = 0;
     for (int q = threadIdx.x; q < G.nlev; q += kBlock) G.levels[q] = 0;
@@ -345,25 +380,42 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
       }
     __syncthreads();
   }
-  // exits become the tile's nodes
+  // exits become the tile's nodes, numbered in border-position order: the
+  // first four waves take one border position per lane
+  constexpr int kSlotWaves = 4 * kSide / kWave;
+  int slot_cell = -1, slot_rank = 0;
+  uint64_t slot_mask = 0;
+  if (threadIdx.x < 4 * kSide) {
+    slot_cell = border_cell(threadIdx.x, b);
+    const bool ex = slot_cell >= 0 && nx[slot_cell] == kTermExit;
+    slot_mask = __ballot(ex);
+    const int lane = threadIdx.x & (kWave - 1);
+    slot_rank = ex ? __popcll(slot_mask & ((uint64_t{1} << lane) - 1)) : -1;
+    if (lane == 0) wave_nodes[threadIdx.x / kWave] = __popcll(slot_mask);
+  }
 #pragma unroll
-  for (int j = 0; j < kPer; j++) {
+  for (int j = 0; j < kPer; j++) {   // the marks on the exits' receivers
     const int i = j * kBlock + threadIdx.x;
-    const bool ex = nx[i] == kTermExit;
-    const uint64_t m = __ballot(ex);
+    if (nx[i] != kTermExit) continue;
+    const size_t g = static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide;
+    h_new[g + dir_offset(rr[j], W)] = __longlong_as_double(static_cast<long long>(kUnsolved));
+  }
+  __syncthreads();
+  int nnode = 0;
+  if (threadIdx.x < 4 * kSide) {
     int base = 0;
-    if ((threadIdx.x & (kWave - 1)) == 0 && m) base = atomicAdd(&nnode, __popcll(m));
-    base = __shfl(base, 0);
-    if (ex) {
-      const int lane = threadIdx.x & (kWave - 1);
-      const int k = base + __popcll(m & ((uint64_t{1} << lane) - 1));
-      nid[i] = static_cast<uint16_t>(k);
+#pragma unroll
+    for (int q = 0; q < kSlotWaves; q++) {
+      base += q < static_cast<int>(threadIdx.x) / kWave ? wave_nodes[q] : 0;
+      nnode += wave_nodes[q];
+    }
+    if (slot_rank >= 0) {
+      const int k = base + slot_rank;
+      slot_node[threadIdx.x] = static_cast<uint16_t>(k);
       const int id = t * kNodeCap + k;
-      const int32_t g = (b.y0 + i / kSide) * W + b.x0 + i % kSide;
-      node_cell[id] = g;
-      node_word[id] = lc[i];
+      node_cell[id] = (b.y0 + slot_cell / kSide) * W + b.x0 + slot_cell % kSide;
+      node_word[id] = lc[slot_cell];
       node_kids[id] = 0;
-      h_new[g + dir_offset(rr[j], W)] = __longlong_as_double(static_cast<long long>(kUnsolved));
     }
   }
   __syncthreads();
@@ -378,9 +430,12 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
     dinfo[g] = static_cast<uint32_t>(cur[j]) | (static_cast<uint32_t>(hops[j]) << 12) |
                (exit_tail ? kInfoExit : 0u);
     if (lx == 0 || ly == 0 || lx == b.tw - 1 || ly == b.th - 1)
-      link[g] = exit_tail ? static_cast<uint32_t>(t * kNodeCap + nid[cur[j]]) : kNoNode;
+      link[g] = exit_tail ? static_cast<uint32_t>(t * kNodeCap +
+                                                  slot_node[border_pos(cur[j] % kSide,
+                                                                       cur[j] / kSide, b)])
+                          : kNoNode;
   }
-  if (threadIdx.x == 0) nexit[t] = nnode;
+  if (threadIdx.x == 0) nexit[t] = nnode;   // every slot thread summed the same count
   // the cells with a root tail, sorted by hops, start the tile's cell list
   // (jw is free now: its words count the cells per hops)
   for (int q = threadIdx.x; q <= top; q += kBlock) jw[q] = 0;
@@ -635,15 +690,6 @@ struct CountShared {
 };
 
 static_assert(sizeof(int) * kKeyCap >= sizeof(uint16_t) * kCells, "list positions fit the keys");
-
-// Border position of a border cell of the tile: rows 0 and th-1, then
-// columns 0 and tw-1 (exits lie on the border; a corner takes its row's).
-__device__ __forceinline__ int border_pos(int lx, int ly, const Box &b) {
-  if (ly == 0) return lx;
-  if (ly == b.th - 1) return kSide + lx;
-  if (lx == 0) return 2 * kSide + ly;
-  return 3 * kSide + ly;
-}
 
 __device__ __forceinline__ int tier_rank(const int *sorted, int n, int tier) {
   int lo = 0, hi = n - 1;
