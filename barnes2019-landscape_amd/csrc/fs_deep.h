@@ -73,7 +73,10 @@ constexpr int kStErr = 0, kStMaxTier = 1, kStItems = 2, kStQueue = 3, kStWords =
 // workgroups of one XCD use their own) to keep tens of thousands of atomics
 // per step off a few addresses
 constexpr int kShards = 64;
-constexpr int kBatch = 4;                 // work items a workgroup takes at once
+// work items a workgroup takes at once: one, since a workgroup solves its
+// items one after another and an item held behind another may be the one
+// that others wait for
+constexpr int kBatch = 1;
 constexpr int32_t kErrTier = 1, kErrChain = 2;
 // the "not yet eroded" mark: a signalling NaN that no arithmetic produces
 constexpr unsigned long long kUnsolved = 0x7FF4A5A5A5A5A5A5ull;
