@@ -877,7 +877,7 @@ int graph_path(fs_model *m, hipEvent_t *ev) {
   hipLaunchKernelGGL(deep::k_deep_clear, dim3(1), dim3(64), 0, s, m->tstat);
   FS_TRY(check_launch());
   FS_TRY(kt_begin(m, FS_K_GRAPH_LOCAL));
-  hipLaunchKernelGGL(deep::k_deep_local, dim3(tiles), dim3(deep::kBlock), 0, s, w, m->rcv, m->cnt,
+  hipLaunchKernelGGL(deep::k_deep_local, dim3(tiles), dim3(deep::kLocalBlock), 0, s, w, m->rcv, m->cnt,
                      m->dinfo, m->link, m->node_word, m->node_kids, m->node_cell, m->nexit,
                      m->tile_n0, m->cells, m->h_alt, m->tstat);
   FS_TRY(check_launch());

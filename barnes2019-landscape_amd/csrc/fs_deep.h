@@ -231,8 +231,9 @@ __device__ __forceinline__ int block_max(int v, int *slot) {
   return *slot;
 }
 
-// Exclusive scan of v over the workgroup (kBlock threads); *total gets the
-// sum.  sh holds kBlock / kWave + 1 ints.  Ends with a barrier.
+// Exclusive scan of v over the workgroup (BLOCK threads); *total gets the
+// sum.  sh holds BLOCK / kWave + 1 ints.  Ends with a barrier.
+template <int BLOCK = kBlock>
 __device__ __forceinline__ int block_scan(int v, int *sh, int *total) {
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   int x = v;
@@ -245,16 +246,16 @@ __device__ __forceinline__ int block_scan(int v, int *sh, int *total) {
   __syncthreads();
   if (threadIdx.x == 0) {
     int run = 0;
-    for (int w = 0; w < kBlock / kWave; w++) {
+    for (int w = 0; w < BLOCK / kWave; w++) {
       const int c = sh[w];
       sh[w] = run;
       run += c;
     }
-    sh[kBlock / kWave] = run;
+    sh[BLOCK / kWave] = run;
   }
   __syncthreads();
   const int r = sh[wave] + x - v;
-  *total = sh[kBlock / kWave];
+  *total = sh[BLOCK / kWave];
   __syncthreads();
   return r;
 }
@@ -294,9 +295,13 @@ __device__ __forceinline__ int border_cell(int k, const Box &b) {
 }
 
 // ---------------------------------------------------------------------------
-// k_deep_local: one tile per workgroup
+// k_deep_local: one tile per workgroup of kLocalBlock threads (half the
+// other tile kernels': its LDS fits three tiles per CU, which 1024-thread
+// workgroups cannot use)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_deep_local(
+constexpr int kLocalBlock = 512;
+constexpr int kLocalPer = kCells / kLocalBlock;
+__global__ __launch_bounds__(kLocalBlock) void k_deep_local(
     GraphWork w, const uint8_t *__restrict__ rcv0, uint32_t *__restrict__ cnt0,
     uint32_t *__restrict__ dinfo0, uint32_t *__restrict__ link0, uint64_t *__restrict__ node_word,
     uint32_t *__restrict__ node_kids, int32_t *__restrict__ node_cell,
@@ -307,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
   __shared__ uint32_t lc[kCells];
   __shared__ uint16_t slot_node[4 * kSide];   // node number of the exit at each border position
   __shared__ int wave_nodes[4 * kSide / kWave];
-  __shared__ int maxhops, scan[kBlock / kWave + 1];
+  __shared__ int maxhops, scan[kLocalBlock / kWave + 1];
   const int wi = blockIdx.x;   // one tile per workgroup; the grid covers the list exactly
   if (wi >= w.size[1]) return;
   const WorkTile wt = work_tile(w, wi);
@@ -321,12 +326,12 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
   if (threadIdx.x == 0) maxhops = 0;
   if (wt.t == 0 && G.levels) {   // the grid's level counts start from zero
     if (threadIdx.x == 0) G.hdr[0] = 0;
-    for (int q = threadIdx.x; q < G.nlev; q += kBlock) G.levels[q] = 0;
+    for (int q = threadIdx.x; q < G.nlev; q += kLocalBlock) G.levels[q] = 0;
   }
-  uint8_t rr[kPer];   // receiver bytes, kept for the exits' marks
+  uint8_t rr[kLocalPer];   // receiver bytes, kept for the exits' marks
 #pragma unroll
-  for (int j = 0; j < kPer; j++) {
-    const int i = j * kBlock + threadIdx.x;
+  for (int j = 0; j < kLocalPer; j++) {
+    const int i = j * kLocalBlock + threadIdx.x;
     const int lx = i % kSide, ly = i / kSide;
     uint16_t n = kTermNone;
     rr[j] = 0xFF;
@@ -346,8 +351,8 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
   for (int round = 0; round <= 12; round++) {
     bool moved = false;
 #pragma unroll
-    for (int j = 0; j < kPer; j++) {
-      const int i = j * kBlock + threadIdx.x;
+    for (int j = 0; j < kLocalPer; j++) {
+      const int i = j * kLocalBlock + threadIdx.x;
       const uint32_t w = jw[i];
       const int a = static_cast<int>(w & 0xFFFFu);
       if (a == i) continue;
@@ -362,11 +367,11 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
     }
   }
   if (bad && threadIdx.x == 0) atomicOr(&tstat[kStErr], kErrChain);   // a receiver cycle
-  int cur[kPer], hops[kPer];
+  int cur[kLocalPer], hops[kLocalPer];
   int mh = 0;
 #pragma unroll
-  for (int j = 0; j < kPer; j++) {
-    const uint32_t w = jw[j * kBlock + threadIdx.x];
+  for (int j = 0; j < kLocalPer; j++) {
+    const uint32_t w = jw[j * kLocalBlock + threadIdx.x];
     cur[j] = static_cast<int>(w & 0xFFFu);
     hops[j] = static_cast<int>(w >> 16) & 0xFFF;
     mh = max(mh, hops[j]);
@@ -376,9 +381,9 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
   // further from the tail, so their counts are complete when it adds its own
   for (int L = top; L >= 1; L--) {
 #pragma unroll
-    for (int j = 0; j < kPer; j++)
+    for (int j = 0; j < kLocalPer; j++)
       if (hops[j] == L) {
-        const int i = j * kBlock + threadIdx.x;
+        const int i = j * kLocalBlock + threadIdx.x;
         atomicAdd(&lc[nx[i]], lc[i]);
       }
     __syncthreads();
@@ -397,8 +402,8 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
     if (lane == 0) wave_nodes[threadIdx.x / kWave] = __popcll(slot_mask);
   }
 #pragma unroll
-  for (int j = 0; j < kPer; j++) {   // the marks on the exits' receivers
-    const int i = j * kBlock + threadIdx.x;
+  for (int j = 0; j < kLocalPer; j++) {   // the marks on the exits' receivers
+    const int i = j * kLocalBlock + threadIdx.x;
     if (nx[i] != kTermExit) continue;
     const size_t g = static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide;
     h_new[g + dir_offset(rr[j], W)] = __longlong_as_double(static_cast<long long>(kUnsolved));
@@ -423,8 +428,8 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kPer; j++) {
-    const int i = j * kBlock + threadIdx.x;
+  for (int j = 0; j < kLocalPer; j++) {
+    const int i = j * kLocalBlock + threadIdx.x;
     if (nx[i] == kTermNone) continue;
     const int lx = i % kSide, ly = i / kSide;
     const size_t g = static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx;
@@ -441,32 +446,32 @@ __global__ __launch_bounds__(kBlock) void k_deep_local(
   if (threadIdx.x == 0) nexit[t] = nnode;   // every slot thread summed the same count
   // the cells with a root tail, sorted by hops, start the tile's cell list
   // (jw is free now: its words count the cells per hops)
-  for (int q = threadIdx.x; q <= top; q += kBlock) jw[q] = 0;
+  for (int q = threadIdx.x; q <= top; q += kLocalBlock) jw[q] = 0;
   __syncthreads();
-  bool rooted[kPer];
+  bool rooted[kLocalPer];
 #pragma unroll
-  for (int j = 0; j < kPer; j++) {
-    const int i = j * kBlock + threadIdx.x;
+  for (int j = 0; j < kLocalPer; j++) {
+    const int i = j * kLocalBlock + threadIdx.x;
     rooted[j] = nx[i] != kTermNone && nx[cur[j]] == kTermRoot;
     if (rooted[j]) atomicAdd(&jw[hops[j]], 1u);
   }
   __syncthreads();
   int carry = 0;
-  for (int base = 0; base <= top; base += kBlock) {
+  for (int base = 0; base <= top; base += kLocalBlock) {
     const int q = base + threadIdx.x;
     const int v = q <= top ? static_cast<int>(jw[q]) : 0;
     int total;
-    const int ex = block_scan(v, scan, &total);
+    const int ex = block_scan<kLocalBlock>(v, scan, &total);
     if (q <= top) jw[q] = static_cast<uint32_t>(carry + ex);
     carry += total;
   }
   __syncthreads();   // every cursor is set before any thread takes a slot
   uint32_t *const list = cells + static_cast<size_t>(t) * kCells;
 #pragma unroll
-  for (int j = 0; j < kPer; j++)
+  for (int j = 0; j < kLocalPer; j++)
     if (rooted[j])
       list[atomicAdd(&jw[hops[j]], 1u)] =
-          static_cast<uint32_t>(j * kBlock + threadIdx.x) | kNoPos << kEntShift;
+          static_cast<uint32_t>(j * kLocalBlock + threadIdx.x) | kNoPos << kEntShift;
   // n0, and the tile's largest hops (which bounds the sort keys of k_deep_counts)
   if (threadIdx.x == 0) tile_n0[t] = carry | top << 16;
 }

@@ -473,7 +473,7 @@ int graph_launches(fs_ensemble *e, const deep::GraphWork &w, int T) {
   const double *h_old = e->h[e->cur];
   double *h_new = e->h[e->cur ^ 1];
   hipLaunchKernelGGL(deep::k_deep_clear, dim3(1), dim3(64), 0, s, e->tstat);
-  hipLaunchKernelGGL(deep::k_deep_local, dim3(T), dim3(deep::kBlock), 0, s, w, e->rcv, e->cnt,
+  hipLaunchKernelGGL(deep::k_deep_local, dim3(T), dim3(deep::kLocalBlock), 0, s, w, e->rcv, e->cnt,
                      e->dinfo, e->link, e->node_word, e->node_kids, e->node_cell, e->nexit,
                      e->tile_n0, e->cells_list, h_new, e->tstat);
   hipLaunchKernelGGL(deep::k_deep_links, dim3(T), dim3(deep::kNodeCap), 0, s, w, e->rcv, e->link,

@@ -46,6 +46,12 @@ KERNEL_BYTES = {
     # uplift + erosion 37
     "k_tile_acc": 33,
     "k_tile_erode": 37,
+    # tile-graph path (drainage of any depth), per cell: k_deep_local reads
+    # the receiver byte and writes the local count and the tail/hops word;
+    # k_deep_counts reads those, the receiver and the elevation, writes the
+    # count and, for the root-tailed cells (most of a tile), the new elevation
+    "k_deep_local": 9,
+    "k_deep_counts": 29,
 }
 
 
@@ -448,7 +454,8 @@ def main():
     paths.update(m.tile_stats())
     n0 = int(levels[1]) if levels.size > 1 else N
     units = {"k_receivers": (W - 4) * (H - 4), "k_topology": N, "k_bfs": N - n0,
-             "k_flowacc": N, "k_erode": N - n0, "k_tile_acc": N, "k_tile_erode": N}
+             "k_flowacc": N, "k_erode": N - n0, "k_tile_acc": N, "k_tile_erode": N,
+             "k_deep_local": N, "k_deep_counts": N}
     dom = max(stats, key=lambda k: stats[k][0])
     ms, launches = stats[dom]
     per_step_bytes = KERNEL_BYTES.get(dom, 0) * units.get(dom, N)
