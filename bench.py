@@ -62,7 +62,7 @@ KERNEL_BYTES = {
 MEASURED_TRAFFIC_PER_CELL = {
     "k_tile_erode": 23.89,
     "k_tile_acc": 8.61,
-    "k_receivers": 10.68,
+    "k_receivers": 10.42,
 }
 
 
