@@ -8,7 +8,9 @@ does not rest on extrapolation:
     the oracle after three GPU steps -- receivers, donor lists, the stack,
     levels[], accumulation and elevations compared bit for bit -- and a
     300-step run of the tile path against the level path through the switch
-    from the 16- to the 32-cell tile halo (about step 261);
+    from the 16- to the 32-cell tile halo (about step 261); and, with the
+    reference's own constants, 64 steps of the default path (the tile graph
+    from step 10, 180+ levels) against the level path;
   * the ensemble workload (256 realizations, sides 512..2048, sampled K, U,
     m): every realization's elevations and levels[] equal those of a
     single-grid model with its size, parameters and seed at steps 10, 20
@@ -95,6 +97,33 @@ def test_headline_grid_tile_path_equals_level_path_through_halo_switch():
         ta = a.tile_stats()
         assert a.path_stats()["tile"] == 300 and b.path_stats() == {"tile": 0, "levels": 300, "graph": 0}
         assert ta["halo16"] > 200 and ta["halo32"] > 0, ta
+
+
+def test_headline_grid_reference_preset_graph_path_equals_level_path():
+    """The reference's own constants (fastscape_RB.cpp:58-65) at 10000^2:
+    drainage passes 32 levels within ten steps, so the default path moves
+    from the halo tiles to the tile graph (180+ levels by step 60); it stays
+    bit-identical to the level path, including the readouts rebuilt from the
+    graph steps' receivers."""
+    W = H = 10000
+    note("10000^2 reference preset: default path vs level path, 64 steps")
+    ref = fa.preset(fa.PRESET_REFERENCE)
+    with fa.FastScape(W, H, ref, engine_seed=0) as a, \
+            fa.FastScape(W, H, ref, engine_seed=0) as b:
+        b.set_path(fa.PATH_LEVELS)
+        done = 0
+        for upto in (8, 30, 64):
+            a.step(upto - done)
+            b.step(upto - done)
+            done = upto
+            note("10000^2 reference preset at step %d: paths %s, graph %s"
+                 % (upto, a.path_stats(), a.graph_stats()))
+            np.testing.assert_array_equal(a.getH(), b.getH(), err_msg="h step %d" % upto)
+            np.testing.assert_array_equal(a.accum(), b.accum(), err_msg="accum %d" % upto)
+            np.testing.assert_array_equal(a.levels(), b.levels(), err_msg="levels %d" % upto)
+        pa = a.path_stats()
+        assert pa["graph"] > 50 and pa["levels"] == 0, pa
+        assert b.levels().size - 3 > 150
 
 
 # ---------------------------------------------------------------------------
