@@ -296,7 +296,7 @@ __device__ __forceinline__ int border_cell(int k, const Box &b) {
 
 // ---------------------------------------------------------------------------
 // k_deep_local: one tile per workgroup of kLocalBlock threads (half the
-// other tile kernels': its LDS fits three tiles per CU, which 1024-thread
+// other tile kernels': its LDS fits four tiles per CU, which 1024-thread
 // workgroups cannot use)
 // ---------------------------------------------------------------------------
 constexpr int kLocalBlock = 512;
@@ -309,7 +309,10 @@ __global__ __launch_bounds__(kLocalBlock) void k_deep_local(
     double *__restrict__ h_new0, int32_t *__restrict__ tstat) {
   __shared__ uint16_t nx[kCells];
   __shared__ uint32_t jw[kCells];    // pointer jumping: ancestor | hops to it << 16
-  __shared__ uint32_t lc[kCells];
+  // local counts: at most kCells, so 16 bits; added through the 32-bit word
+  // that holds a pair (no carry crosses halves), which keeps the tile's LDS
+  // small enough for four workgroups per CU
+  __shared__ __attribute__((aligned(4))) uint16_t lc[kCells];
   __shared__ uint16_t slot_node[4 * kSide];   // node number of the exit at each border position
   __shared__ int wave_nodes[4 * kSide / kWave];
   __shared__ int maxhops, scan[kLocalBlock / kWave + 1];
@@ -341,7 +344,7 @@ __global__ __launch_bounds__(kLocalBlock) void k_deep_local(
     }
     nx[i] = n;
     jw[i] = n < kTermNone ? (static_cast<uint32_t>(n) | (1u << 16)) : static_cast<uint32_t>(i);
-    lc[i] = n != kTermNone ? 1u : 0u;
+    lc[i] = n != kTermNone ? 1 : 0;
   }
   __syncthreads();
   // tail and hops of every cell by pointer jumping: each round doubles the
@@ -384,7 +387,9 @@ __global__ __launch_bounds__(kLocalBlock) void k_deep_local(
     for (int j = 0; j < kLocalPer; j++)
       if (hops[j] == L) {
         const int i = j * kLocalBlock + threadIdx.x;
-        atomicAdd(&lc[nx[i]], lc[i]);
+        const int d = nx[i];
+        atomicAdd(reinterpret_cast<uint32_t *>(lc) + (d >> 1),
+                  static_cast<uint32_t>(lc[i]) << ((d & 1) * 16));
       }
     __syncthreads();
   }
