@@ -852,7 +852,7 @@ int launch_graph_erosion(fs_model *m, int tiles) {
   hipStream_t s = m->stream;
   const deep::GraphWork w = graph_work(m);
   FS_TRY(kt_begin(m, FS_K_GRAPH_COUNTS));
-  hipLaunchKernelGGL(deep::k_deep_counts<NMODE>, dim3(tiles), dim3(deep::kBlock), 0, s, w, m->rcv,
+  hipLaunchKernelGGL(deep::k_deep_counts<NMODE>, dim3(tiles), dim3(deep::kCntBlock), 0, s, w, m->rcv,
                      m->cnt, m->dinfo, m->link, m->node_out, m->node_cell, m->node_tier,
                      m->nexit, m->tile_n0, m->cells, m->tcur, m->items, m->ftab, m->h, m->h_alt);
   FS_TRY(check_launch());

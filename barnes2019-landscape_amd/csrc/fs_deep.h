@@ -683,26 +683,50 @@ __global__ __launch_bounds__(kBlock) void k_deep_levels(GraphWork w,
 }
 
 // ---------------------------------------------------------------------------
-// k_deep_counts: one tile per workgroup
+// k_deep_counts: one tile per workgroup of kCntBlock threads
 // ---------------------------------------------------------------------------
-constexpr int kKeyCap = 3072;    // (tier rank, hops) sort keys held apart from the elevations
+// Half the threads of the other tile kernels: with the time-shared region
+// below the tile's LDS is a third of a CU's, so three tiles run per CU and
+// one tile's loads overlap another's wavefront.
+constexpr int kCntBlock = 512;
+constexpr int kCntPer = kCells / kCntBlock;
+constexpr int kCntWaves = kCntBlock / kWave;
 constexpr int kHashCap = 512;    // open-addressing set of the tile's exit tiers
+// Each cell's step inside the tile: its receiver direction 0..7 when the
+// receiver is a tile cell, else one of these.
+constexpr uint8_t kCodeExit = 8, kCodeRoot = 9, kCodeNone = 10;
 struct CountShared {
-  double hs[kCells];              // elevations, then the root-tailed cells' new ones
+  union {
+    // last: the root-tailed cells' unsolved marks, then their new elevations (bits)
+    uint64_t hs[kCells];
+    struct {
+      int keys[kCells];             // sort cursors, then each cell's list position (16 bits)
+      int tset[kHashCap];
+      int tiers[kTiersPerTile];     // the tile's distinct tiers, then sorted
+      int sorted[kTiersPerTile];
+      int hist[kTiersPerTile];      // cells per tier rank, then their first slot
+      uint16_t tier_b[4 * kSide];   // tier of the exit at each border position
+    } s;
+  } u;
   uint32_t cs[kCells];
-  int keys[kKeyCap];               // sort keys, then each cell's list position
-  uint16_t nx[kCells];
-  uint8_t done[kCells];
-  int tset[kHashCap];
-  uint16_t tier_b[4 * kSide];     // tier of the exit at each border position
-  int tiers[kTiersPerTile];       // the tile's distinct tiers, then sorted
-  int sorted[kTiersPerTile];
-  int hist[kTiersPerTile];        // cells per tier rank, then their cursor
-  int scan[kBlock / kWave + 1];
+  uint8_t code[kCells];
+  int scan[kCntWaves + 1];
   int ntier, maxhops;
 };
+static_assert(sizeof(CountShared) * 3 <= 160 * 1024, "three tiles per CU");
 
-static_assert(sizeof(int) * kKeyCap >= sizeof(uint16_t) * kCells, "list positions fit the keys");
+__device__ __forceinline__ uint8_t tile_code(uint8_t r, int lx, int ly, const Box &b) {
+  if (r >= 8) return kCodeRoot;
+  const int nx = lx + dir_dx(r), ny = ly + dir_dy(r);
+  if (static_cast<unsigned>(nx) >= static_cast<unsigned>(b.tw) ||
+      static_cast<unsigned>(ny) >= static_cast<unsigned>(b.th))
+    return kCodeExit;
+  return r;
+}
+// the tile cell one step from cell i in direction r (a tile code below 8)
+__device__ __forceinline__ int tile_step(int i, int r) {
+  return i + dir_dy(r) * kSide + dir_dx(r);
+}
 
 __device__ __forceinline__ int tier_rank(const int *sorted, int n, int tier) {
   int lo = 0, hi = n - 1;
@@ -715,7 +739,7 @@ __device__ __forceinline__ int tier_rank(const int *sorted, int n, int tier) {
 }
 
 template <int NMODE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_deep_counts(
+__global__ __launch_bounds__(kCntBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_deep_counts(
     GraphWork w, const uint8_t *__restrict__ rcv0, uint32_t *__restrict__ cnt0,
     const uint32_t *__restrict__ dinfo0, const uint32_t *__restrict__ link0,
     const uint32_t *__restrict__ node_out, const int32_t *__restrict__ node_cell,
@@ -724,6 +748,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     int32_t *__restrict__ tcur, uint64_t *__restrict__ items, const double *__restrict__ ftab0,
     const double *__restrict__ h_old0, double *__restrict__ h_new0) {
   __shared__ CountShared sh;
+  auto &S = sh.u.s;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int wi = blockIdx.x;   // one tile per workgroup; the grid covers the list exactly
   if (wi >= w.size[1]) return;
@@ -739,29 +764,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
   int32_t *const hdr = G.hdr;
   const int t = wt.tg;   // per-tile arrays and node ids
   const Box b = tile_box(wt.t, G.tiles_x, W, H);
-  const int n0top = uni(tile_n0[t]);
-  const int n0 = n0top & 0xFFFF, top = n0top >> 16;
-  // the elevations now, unless the sort keys may need their array as scratch
-  const bool early_h = top + 1 <= kKeyCap;
-  uint32_t info[kPer];
+  const int n0 = uni(tile_n0[t]) & 0xFFFF;
+  uint32_t info[kCntPer];
   if (threadIdx.x == 0) sh.ntier = sh.maxhops = 0;
-  if (threadIdx.x < kTiersPerTile) sh.hist[threadIdx.x] = 0;
-  for (int q = threadIdx.x; q < kHashCap; q += kBlock) sh.tset[q] = -1;
+  if (threadIdx.x < kTiersPerTile) S.hist[threadIdx.x] = 0;
+  for (int q = threadIdx.x; q < kHashCap; q += kCntBlock) S.tset[q] = -1;
 #pragma unroll
-  for (int j = 0; j < kPer; j++) {
-    const int i = j * kBlock + threadIdx.x;
+  for (int j = 0; j < kCntPer; j++) {
+    const int i = j * kCntBlock + threadIdx.x;
     const int lx = i % kSide, ly = i / kSide;
-    uint16_t n = kTermNone;
+    uint8_t c = kCodeNone;
     info[j] = 0;
     if (lx < b.tw && ly < b.th) {
       const size_t g = static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx;
-      n = tile_next(rcv[g], lx, ly, b);
+      c = tile_code(rcv[g], lx, ly, b);
       info[j] = dinfo[g];
       sh.cs[i] = cnt[g];
-      if (early_h) sh.hs[i] = h_old[g];
     }
-    sh.nx[i] = n;
-    sh.done[i] = 0;
+    sh.code[i] = c;
   }
   const int n = nexit[t];
   int my_tier = -1;
@@ -769,7 +789,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const int id = t * kNodeCap + threadIdx.x;
     const int32_t g = node_cell[id];
     my_tier = node_tier[id];
-    sh.tier_b[border_pos(g % W - b.x0, g / W - b.y0, b)] = static_cast<uint16_t>(my_tier);
+    S.tier_b[border_pos(g % W - b.x0, g / W - b.y0, b)] = static_cast<uint16_t>(my_tier);
   }
   __syncthreads();
   // the tile's distinct exit tiers: the node that inserts a tier into the
@@ -777,9 +797,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
   if (my_tier >= 0) {
     int slot = static_cast<int>((static_cast<uint32_t>(my_tier) * 2654435761u) >> 23);
     for (;;) {   // at most kNodeCap tiers in kHashCap slots: a free slot is always found
-      const int old = atomicCAS(&sh.tset[slot], -1, my_tier);
+      const int old = atomicCAS(&S.tset[slot], -1, my_tier);
       if (old == -1) {
-        sh.sorted[atomicAdd(&sh.ntier, 1)] = my_tier;
+        S.sorted[atomicAdd(&sh.ntier, 1)] = my_tier;
         break;
       }
       if (old == my_tier) break;
@@ -818,9 +838,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
             int cur = ty * kSide + tx;
             for (int hop = 0; hop < kCells; hop++) {
               atomicAdd(&sh.cs[cur], v);
-              const uint16_t nn = sh.nx[cur];
-              if (nn >= kTermNone) break;
-              cur = nn;
+              const int c = sh.code[cur];
+              if (c >= 8) break;
+              cur = tile_step(cur, c);
             }
           }
         }
@@ -830,29 +850,29 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
   __syncthreads();
   const int ntier = sh.ntier;
   if (static_cast<int>(threadIdx.x) < ntier) {
-    const int v = sh.sorted[threadIdx.x];
+    const int v = S.sorted[threadIdx.x];
     int rank = 0;
-    for (int q = 0; q < ntier; q++) rank += sh.sorted[q] < v;
-    sh.tiers[rank] = v;
+    for (int q = 0; q < ntier; q++) rank += S.sorted[q] < v;
+    S.tiers[rank] = v;
   }
   __syncthreads();
   // final counts; each exit-tailed cell's tier rank
-  int rank[kPer];
+  int rank[kCntPer];
 #pragma unroll
-  for (int j = 0; j < kPer; j++) {
-    const int i = j * kBlock + threadIdx.x;
+  for (int j = 0; j < kCntPer; j++) {
+    const int i = j * kCntBlock + threadIdx.x;
     rank[j] = -1;
-    if (sh.nx[i] == kTermNone) continue;
+    if (sh.code[i] == kCodeNone) continue;
     cnt[static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide] = sh.cs[i];
     if (!(info[j] & kInfoExit)) continue;
     const int tl = info_tail(info[j]);
-    rank[j] = tier_rank(sh.tiers, ntier, sh.tier_b[border_pos(tl % kSide, tl / kSide, b)]);
-    atomicAdd(&sh.hist[rank[j]], 1);
+    rank[j] = tier_rank(S.tiers, ntier, S.tier_b[border_pos(tl % kSide, tl / kSide, b)]);
+    atomicAdd(&S.hist[rank[j]], 1);
   }
   __syncthreads();
   int mh = 0;
 #pragma unroll
-  for (int j = 0; j < kPer; j++)
+  for (int j = 0; j < kCntPer; j++)
     if (rank[j] >= 0) mh = max(mh, info_hops(info[j]));
   const int span = block_max(mh, &sh.maxhops) + 1;   // hops values per tier
   // the work items: tier ranks in order, each its cells' count; the slot
@@ -861,104 +881,108 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
   int item_pos = -1;
   uint64_t item_word = 0;
   {
-    const int v = static_cast<int>(threadIdx.x) < ntier ? sh.hist[threadIdx.x] : 0;
+    const int v = static_cast<int>(threadIdx.x) < ntier ? S.hist[threadIdx.x] : 0;
     int total;
-    const int ex = block_scan(v, sh.scan, &total);
+    const int ex = block_scan<kCntBlock>(v, sh.scan, &total);
     if (static_cast<int>(threadIdx.x) < ntier) {
-      const int tier = sh.tiers[threadIdx.x];
+      const int tier = S.tiers[threadIdx.x];
       item_pos = atomicAdd(&tcur[tier * kShards + (t & (kShards - 1))], 1);
       item_word = item_pack(wi, n0 + ex, v);
-      sh.hist[threadIdx.x] = n0 + ex;   // first slot of the rank
+      S.hist[threadIdx.x] = n0 + ex;   // first slot of the rank
     }
   }
   // each rank's cells sorted by hops (a work item solves its cells in list
   // order): one counting sort over (rank, hops) keys when they fit the key
-  // array, else one per rank; hops beyond the key array (only when the
-  // elevations were not loaded yet) count in the elevation array
-  const bool big = span > kKeyCap;
-  int *const keyc = big ? reinterpret_cast<int *>(sh.hs) : sh.keys;
+  // array, else one per rank (hops are below kCells, so one rank's always fit)
   uint32_t *const list = cells + static_cast<size_t>(t) * kCells;
-  const int passes = ntier * span <= kKeyCap ? 1 : ntier;
-  int lp[kPer];   // each exit-tailed cell's position in the list
+  const int passes = ntier * span <= kCells ? 1 : ntier;
+  int lp[kCntPer];   // each exit-tailed cell's position in the list
 #pragma unroll
-  for (int j = 0; j < kPer; j++) lp[j] = -1;
+  for (int j = 0; j < kCntPer; j++) lp[j] = -1;
   for (int pass = 0; pass < passes; pass++) {
     const int nkeys = passes == 1 ? ntier * span : span;
-    for (int q = threadIdx.x; q < nkeys; q += kBlock) keyc[q] = 0;
+    for (int q = threadIdx.x; q < nkeys; q += kCntBlock) S.keys[q] = 0;
     __syncthreads();
-    int key[kPer];
+    int key[kCntPer];
 #pragma unroll
-    for (int j = 0; j < kPer; j++) {
+    for (int j = 0; j < kCntPer; j++) {
       key[j] = -1;
       if (rank[j] < 0 || (passes > 1 && rank[j] != pass)) continue;
       key[j] = (passes == 1 ? rank[j] * span : 0) + info_hops(info[j]);
-      atomicAdd(&keyc[key[j]], 1);
+      atomicAdd(&S.keys[key[j]], 1);
     }
     __syncthreads();
-    int carry = passes == 1 ? n0 : sh.hist[pass];
-    for (int base = 0; base < nkeys; base += kBlock) {
+    int carry = passes == 1 ? n0 : S.hist[pass];
+    for (int base = 0; base < nkeys; base += kCntBlock) {
       const int q = base + threadIdx.x;
-      const int v = q < nkeys ? keyc[q] : 0;
+      const int v = q < nkeys ? S.keys[q] : 0;
       int total;
-      const int ex = block_scan(v, sh.scan, &total);
-      if (q < nkeys) keyc[q] = carry + ex;
+      const int ex = block_scan<kCntBlock>(v, sh.scan, &total);
+      if (q < nkeys) S.keys[q] = carry + ex;
       carry += total;
     }
     __syncthreads();   // every cursor is set before any thread takes a slot
 #pragma unroll
-    for (int j = 0; j < kPer; j++)
-      if (key[j] >= 0) lp[j] = atomicAdd(&keyc[key[j]], 1);
+    for (int j = 0; j < kCntPer; j++)
+      if (key[j] >= 0) lp[j] = atomicAdd(&S.keys[key[j]], 1);
     __syncthreads();
   }
   // the list entries: each cell with the position of its in-tile receiver
   // (same tail, so the same work item, one hop earlier) relative to the item
   {
-    uint16_t *const lpos = reinterpret_cast<uint16_t *>(sh.keys);   // free after the sort
+    uint16_t *const lpos = reinterpret_cast<uint16_t *>(S.keys);   // free after the sort
 #pragma unroll
-    for (int j = 0; j < kPer; j++)
-      if (lp[j] >= 0) lpos[j * kBlock + threadIdx.x] = static_cast<uint16_t>(lp[j]);
+    for (int j = 0; j < kCntPer; j++)
+      if (lp[j] >= 0) lpos[j * kCntBlock + threadIdx.x] = static_cast<uint16_t>(lp[j]);
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kPer; j++) {
+    for (int j = 0; j < kCntPer; j++) {
       if (lp[j] < 0) continue;
-      const int i = j * kBlock + threadIdx.x;
-      const int ri = sh.nx[i];
-      const uint32_t rp = ri < kTermNone
-                              ? static_cast<uint32_t>(lpos[ri] - sh.hist[rank[j]])
-                              : kNoPos;
+      const int i = j * kCntBlock + threadIdx.x;
+      const int c = sh.code[i];
+      const uint32_t rp = c < 8 ? static_cast<uint32_t>(lpos[tile_step(i, c)] - S.hist[rank[j]])
+                                : kNoPos;
       list[lp[j]] = static_cast<uint32_t>(i) | rp << kEntShift;
     }
   }
   if (item_pos >= 0) items[item_pos] = item_word;
-  if (!early_h) {   // the elevations, now that the sort's scratch is free
+  // the cells with a root tail, from the list k_deep_local sorted by hops;
+  // each wave takes 64-cell chunks in order.  A lane takes its cells' old
+  // elevations from memory and solves a cell once its receiver (an earlier
+  // cell of the list) holds its new elevation in place of the unsolved mark,
+  // which one 64-bit LDS read both signals and delivers
+  int ci[kCntPer];
+  double h0s[kCntPer];
 #pragma unroll
-    for (int j = 0; j < kPer; j++) {
-      const int i = j * kBlock + threadIdx.x;
-      if (sh.nx[i] != kTermNone)
-        sh.hs[i] = h_old[static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide];
-    }
-    __syncthreads();
+  for (int c = 0; c < kCntPer; c++) {   // kCntWaves waves cover kCells in kCntPer rounds
+    const int p = (wave + c * kCntWaves) * kWave + lane;
+    ci[c] = p < n0 ? static_cast<int>(list[p] & kEntCell) : -1;
   }
-  // the cells with a root tail, from the list k_deep_local sorted
-  // by hops; each wave takes 64-cell chunks in order and a lane solves its
-  // cell once its receiver (an earlier cell of the list) is done
-  for (int ch = wave; ch * kWave < n0; ch += kBlock / kWave) {
-    const int p = ch * kWave + lane;
-    if (p >= n0) continue;
-    const int i = static_cast<int>(list[p] & kEntCell);
-    const int ri = sh.nx[i];
-    const bool root = ri == kTermRoot;
-    double fact = 0.0;
-    if (!root) {
-      const int d = ri - i;   // a diagonal step is +-63 or +-65 cells away
-      const bool diag = d != 1 && d != -1 && d != kSide && d != -kSide;
-      fact = ftab[2 * static_cast<size_t>(sh.cs[i]) + (diag ? 1 : 0)];
-    }
-    const double h0 = sh.hs[i];
+#pragma unroll
+  for (int c = 0; c < kCntPer; c++)
+    if (ci[c] >= 0)
+      h0s[c] = h_old[static_cast<size_t>(b.y0 + ci[c] / kSide) * W + b.x0 + ci[c] % kSide];
+  __syncthreads();   // the sort's and the entries' arrays are dead: the marks go down
+  uint64_t *const hbits = sh.u.hs;
+#pragma unroll
+  for (int c = 0; c < kCntPer; c++)
+    if (ci[c] >= 0) hbits[ci[c]] = kUnsolved;
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < kCntPer; c++) {
+    const int i = ci[c];
+    if (i < 0) break;   // later rounds lie further down the list
+    const int r = sh.code[i];
+    const bool root = r == kCodeRoot;
+    const int ri = root ? i : tile_step(i, r);
+    const double fact = root ? 0.0 : ftab[2 * static_cast<size_t>(sh.cs[i]) + (r & 1)];
+    const double h0 = h0s[c];
     int spins = 0;
     for (;;) {
-      const bool ready = root || __hip_atomic_load(&sh.done[ri], __ATOMIC_ACQUIRE,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+      const uint64_t rb = root ? 0
+                               : __hip_atomic_load(&hbits[ri], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+      const bool ready = root || rb != kUnsolved;
       const bool give_up = !ready && ++spins > kSpinCap;
       if (ready || give_up) {
         double hn = h0;
@@ -968,11 +992,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
           const int gx = b.x0 + i % kSide, gy = b.y0 + i / kSide;
           if (gx >= 2 && gx < W - 2 && gy >= 2 && gy < H - 2) hn += k.ud;
         } else {
-          hn = solve<NMODE>(G.nmode, h0 + k.ud, sh.hs[ri], fact, k, hdr);
+          hn = solve<NMODE>(G.nmode, h0 + k.ud, __longlong_as_double(static_cast<long long>(rb)),
+                            fact, k, hdr);
         }
-        sh.hs[i] = hn;
-        __hip_atomic_store(&sh.done[i], static_cast<uint8_t>(1), __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_store(&hbits[i], static_cast<uint64_t>(__double_as_longlong(hn)),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         break;
       }
       __builtin_amdgcn_s_sleep(1);
@@ -980,10 +1004,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kPer; j++) {
-    const int i = j * kBlock + threadIdx.x;
-    if (sh.nx[i] == kTermNone || (info[j] & kInfoExit)) continue;
-    h_new[static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide] = sh.hs[i];
+  for (int j = 0; j < kCntPer; j++) {
+    const int i = j * kCntBlock + threadIdx.x;
+    if (sh.code[i] == kCodeNone || (info[j] & kInfoExit)) continue;
+    h_new[static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide] =
+        __longlong_as_double(static_cast<long long>(hbits[i]));
   }
 }
 

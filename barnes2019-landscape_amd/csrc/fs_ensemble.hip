@@ -486,7 +486,7 @@ int graph_launches(fs_ensemble *e, const deep::GraphWork &w, int T) {
                      e->node_kids, e->node_parent, e->nexit, e->node_out, e->tstat);
   hipLaunchKernelGGL(deep::k_deep_levels, dim3(T), dim3(deep::kBlock), 0, s, w, e->dinfo, e->link,
                      e->node_level);
-  hipLaunchKernelGGL(deep::k_deep_counts<NM>, dim3(T), dim3(deep::kBlock), 0, s, w, e->rcv,
+  hipLaunchKernelGGL(deep::k_deep_counts<NM>, dim3(T), dim3(deep::kCntBlock), 0, s, w, e->rcv,
                      e->cnt, e->dinfo, e->link, e->node_out, e->node_cell, e->node_tier, e->nexit,
                      e->tile_n0, e->cells_list, e->tcur, e->items, e->ftab, h_old, h_new);
   hipLaunchKernelGGL(deep::k_deep_erode<NM>, dim3(static_cast<unsigned>(e->graph_blocks)),
