@@ -695,6 +695,7 @@ constexpr int kHashCap = 512;    // open-addressing set of the tile's exit tiers
 // Each cell's step inside the tile: its receiver direction 0..7 when the
 // receiver is a tile cell, else one of these.
 constexpr uint8_t kCodeExit = 8, kCodeRoot = 9, kCodeNone = 10;
+constexpr uint8_t kCodeStep = 15, kCodeExitTail = 16;   // the step; the cell's tail is an exit
 struct CountShared {
   union {
     // last: the root-tailed cells' unsolved marks, then their new elevations (bits)
@@ -777,8 +778,8 @@ __global__ __launch_bounds__(kCntBlock) __attribute__((amdgpu_waves_per_eu(6))) 
     info[j] = 0;
     if (lx < b.tw && ly < b.th) {
       const size_t g = static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx;
-      c = tile_code(rcv[g], lx, ly, b);
       info[j] = dinfo[g];
+      c = tile_code(rcv[g], lx, ly, b) | ((info[j] & kInfoExit) ? kCodeExitTail : 0);
       sh.cs[i] = cnt[g];
     }
     sh.code[i] = c;
@@ -838,7 +839,7 @@ __global__ __launch_bounds__(kCntBlock) __attribute__((amdgpu_waves_per_eu(6))) 
             int cur = ty * kSide + tx;
             for (int hop = 0; hop < kCells; hop++) {
               atomicAdd(&sh.cs[cur], v);
-              const int c = sh.code[cur];
+              const int c = sh.code[cur] & kCodeStep;
               if (c >= 8) break;
               cur = tile_step(cur, c);
             }
@@ -862,7 +863,7 @@ __global__ __launch_bounds__(kCntBlock) __attribute__((amdgpu_waves_per_eu(6))) 
   for (int j = 0; j < kCntPer; j++) {
     const int i = j * kCntBlock + threadIdx.x;
     rank[j] = -1;
-    if (sh.code[i] == kCodeNone) continue;
+    if ((sh.code[i] & kCodeStep) == kCodeNone) continue;
     cnt[static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide] = sh.cs[i];
     if (!(info[j] & kInfoExit)) continue;
     const int tl = info_tail(info[j]);
@@ -939,7 +940,7 @@ __global__ __launch_bounds__(kCntBlock) __attribute__((amdgpu_waves_per_eu(6))) 
     for (int j = 0; j < kCntPer; j++) {
       if (lp[j] < 0) continue;
       const int i = j * kCntBlock + threadIdx.x;
-      const int c = sh.code[i];
+      const int c = sh.code[i] & kCodeStep;
       const uint32_t rp = c < 8 ? static_cast<uint32_t>(lpos[tile_step(i, c)] - S.hist[rank[j]])
                                 : kNoPos;
       list[lp[j]] = static_cast<uint32_t>(i) | rp << kEntShift;
@@ -968,6 +969,11 @@ __global__ __launch_bounds__(kCntBlock) __attribute__((amdgpu_waves_per_eu(6))) 
   for (int c = 0; c < kCntPer; c++)
     if (ci[c] >= 0) hbits[ci[c]] = kUnsolved;
   __syncthreads();
+  auto fact_of = [&](int i) {
+    const int r = sh.code[i];
+    return r == kCodeRoot ? 0.0 : ftab[2 * static_cast<size_t>(sh.cs[i]) + (r & 1)];
+  };
+  double fnext = ci[0] >= 0 ? fact_of(ci[0]) : 0.0;
 #pragma unroll
   for (int c = 0; c < kCntPer; c++) {
     const int i = ci[c];
@@ -975,7 +981,8 @@ __global__ __launch_bounds__(kCntBlock) __attribute__((amdgpu_waves_per_eu(6))) 
     const int r = sh.code[i];
     const bool root = r == kCodeRoot;
     const int ri = root ? i : tile_step(i, r);
-    const double fact = root ? 0.0 : ftab[2 * static_cast<size_t>(sh.cs[i]) + (r & 1)];
+    const double fact = fnext;
+    if (c + 1 < kCntPer && ci[c + 1] >= 0) fnext = fact_of(ci[c + 1]);
     const double h0 = h0s[c];
     int spins = 0;
     for (;;) {
@@ -1006,7 +1013,8 @@ __global__ __launch_bounds__(kCntBlock) __attribute__((amdgpu_waves_per_eu(6))) 
 #pragma unroll
   for (int j = 0; j < kCntPer; j++) {
     const int i = j * kCntBlock + threadIdx.x;
-    if (sh.code[i] == kCodeNone || (info[j] & kInfoExit)) continue;
+    const int c = sh.code[i];
+    if ((c & kCodeStep) == kCodeNone || (c & kCodeExitTail)) continue;
     h_new[static_cast<size_t>(b.y0 + i / kSide) * W + b.x0 + i % kSide] =
         __longlong_as_double(static_cast<long long>(hbits[i]));
   }
