@@ -154,6 +154,7 @@ struct fs_model {
   uint64_t *items = nullptr;       // work items in tier order
   int32_t *tstat = nullptr;        // error bits, deepest tier, items, queue head
   int graph_blocks = 0;            // persistent erosion grid
+  int graph_queues = 1;            // its work-item queues
   deep::GraphGrid *graph_grid = nullptr;   // this model as the tile-graph kernels' one grid
   int32_t *graph_list = nullptr;           // its work list: first[2], which[1], size[2]
   uint64_t graph_steps = 0;
@@ -799,6 +800,7 @@ int ensure_graph(fs_model *m) {
   m->graph_blocks = m->nmode == 1   ? deep::erode_grid<1>(ncu)
                     : m->nmode == 2 ? deep::erode_grid<2>(ncu)
                                     : deep::erode_grid<0>(ncu);
+  m->graph_queues = deep::erode_queues(m->graph_blocks);
   FS_TRY(fs::dev_alloc(&m->dinfo, N));
   FS_TRY(fs::dev_alloc(&m->link, N));
   FS_TRY(fs::dev_alloc(&m->node_word, nodes));
@@ -860,7 +862,7 @@ int launch_graph_erosion(fs_model *m, int tiles) {
   FS_TRY(kt_begin(m, FS_K_GRAPH_ERODE));
   hipLaunchKernelGGL(deep::k_deep_erode<NMODE>, dim3(static_cast<unsigned>(m->graph_blocks)),
                      dim3(deep::kEroBlock), 0, s, w, m->rcv, m->cnt, m->cells, m->items, m->ftab,
-                     m->h, m->h_alt, m->tstat);
+                     m->h, m->h_alt, m->tstat, m->graph_queues);
   FS_TRY(check_launch());
   return kt_end(m);
 }
@@ -893,7 +895,7 @@ int graph_path(fs_model *m, hipEvent_t *ev) {
                      static_cast<uint32_t *>(nullptr), m->thist, m->tstat);
   FS_TRY(check_launch());
   hipLaunchKernelGGL(deep::k_deep_scan, dim3(1), dim3(deep::kBlock), 0, s, m->thist, m->tcur,
-                     m->tstat);
+                     m->tstat, m->graph_queues);
   FS_TRY(check_launch());
   FS_TRY(mark(ev[3], s));
   hipLaunchKernelGGL(deep::k_deep_sum, dim3(tiles), dim3(deep::kNodeCap), 0, s, w, m->node_word,

@@ -23,7 +23,8 @@
 //   k_deep_links  every node's parent, and the number of children per node
 //   k_deep_tiers  every node's tier (walking its parents), and the number of
 //                 (tile, tier) work items per tier
-//   k_deep_scan   the work items' offsets by tier (one workgroup)
+//   k_deep_scan   the work items' offsets: up to 64 queues, each holding its
+//                 tiles' items in tier order (one workgroup)
 //   k_deep_sum    the count through every exit: its local count plus its
 //                 children's, leaf to root (the last child to arrive carries
 //                 the sum on, so each node is summed once)
@@ -33,8 +34,9 @@
 //                 (tier 0) in LDS, from the roots up; the tile's other cells
 //                 grouped by tier into the cell list, one work item per tier
 //                 placed in tier order
-//   k_deep_erode  a persistent grid takes the work items (tiers 1 and up) in
-//                 tier order and erodes each item's cells from its exits up:
+//   k_deep_erode  a persistent grid takes the work items (tiers 1 and up)
+//                 from the queues, each in tier order, and erodes each item's
+//                 cells from its exits up:
 //                 an exit waits until its receiver's new elevation replaces
 //                 the mark; a wave's cells wait on their receivers' done
 //                 flags in LDS
@@ -44,9 +46,10 @@
 // after its receiver with the reference's Newton iteration (RB:306-328) from
 // the same inputs (its elevation plus uplift, the receiver's new elevation,
 // the factor of its count), so the new elevations are the reference's bits.
-// Work items wait only on items of the tier below, which the queue handed
-// out earlier to running workgroups, so the persistent grid always drains;
-// every wait is bounded all the same and reported in hdr[2].
+// Work items wait only on items of lower tiers, which their queues hand out
+// before any item of a higher tier to workgroups that are running, so the
+// persistent grid always drains; every wait is bounded all the same and
+// reported in hdr[2].
 #ifndef FS_DEEP_H
 #define FS_DEEP_H
 
@@ -67,16 +70,19 @@ constexpr int kSumShift = 40;             // node word: arrivals << 40 | count
 constexpr uint64_t kSumMask = (uint64_t{1} << kSumShift) - 1;
 constexpr uint64_t kOneArrival = uint64_t{1} << kSumShift;
 constexpr int kTierCap = FS_MAX_LEVELS;   // a tier is at most the depth of the drainage
-// tstat words
-constexpr int kStErr = 0, kStMaxTier = 1, kStItems = 2, kStQueue = 3, kStWords = 4;
-// per-tier item counters are split over kShards words (by tile, so the
-// workgroups of one XCD use their own) to keep tens of thousands of atomics
-// per step off a few addresses
+// per-tier item counters are split over kShards words (by tile) to keep
+// tens of thousands of atomics per step off a few addresses; the work items
+// are handed out by up to kShards queues for the same reason
 constexpr int kShards = 64;
-// work items a workgroup takes at once: one, since a workgroup solves its
-// items one after another and an item held behind another may be the one
-// that others wait for
-constexpr int kBatch = 1;
+// tstat words: error bits, deepest tier, items; the first item of each
+// queue (and the end of the last); each queue's cursor on a cache line of
+// its own (kQStride words apart).  Steps clear the first kStClear words.
+constexpr int kStErr = 0, kStMaxTier = 1, kStItems = 2, kStClear = 3;
+constexpr int kStQStart = 4;
+constexpr int kQStride = 32;
+constexpr int kStQCur = 128;
+constexpr int kStWords = kStQCur + kShards * kQStride;
+static_assert(kStQStart + kShards + 1 <= kStQCur, "queue starts fit before the cursors");
 constexpr int32_t kErrTier = 1, kErrChain = 2;
 // the "not yet eroded" mark: a signalling NaN that no arithmetic produces
 constexpr unsigned long long kUnsolved = 0x7FF4A5A5A5A5A5A5ull;
@@ -553,30 +559,45 @@ __global__ __launch_bounds__(kNodeCap) void k_deep_tiers(
   }
 }
 
-// Offsets of the work items by tier: thist (counts per tier and shard) ->
-// tcur (first slot of each, then the cursor k_deep_counts appends at), so
-// the items of a tier are contiguous and the tiers in order; clears thist
-// for the next step and the queue head.  One workgroup of kBlock threads.
+// Offsets of the work items: thist (counts per tier and shard) -> tcur
+// (the first slot of each (tier, shard), then the cursor k_deep_counts
+// appends at).  The items are laid out queue by queue -- queue q holds the
+// shards s with s % nq == q -- and each queue's tier by tier, so a queue
+// hands out its items in tier order.  Writes each queue's first item and
+// cursor, clears thist for the next step.  One workgroup of kBlock threads;
+// nq divides kShards.
 __global__ __launch_bounds__(kBlock) void k_deep_scan(int32_t *__restrict__ thist,
                                                       int32_t *__restrict__ tcur,
-                                                      int32_t *__restrict__ tstat) {
+                                                      int32_t *__restrict__ tstat, int nq) {
   __shared__ int sh[kBlock / kWave + 1];
-  const int n = (tstat[kStMaxTier] + 1) * kShards;   // tier-major, shard-minor
+  const int spq = kShards / nq;                       // shards per queue
+  const int per_q = (tstat[kStMaxTier] + 1) * spq;    // (tier, shard) words per queue
+  const int n = nq * per_q;
   int carry = 0;
   for (int base = 0; base < n; base += kBlock) {
     const int k = base + threadIdx.x;
-    const int v = k < n ? thist[k] : 0;
+    int idx = 0, v = 0;
+    if (k < n) {
+      const int q = k / per_q, rem = k - q * per_q;
+      const int tier = rem / spq, s = q + (rem - tier * spq) * nq;
+      idx = tier * kShards + s;
+      v = thist[idx];
+    }
     int total;
     const int ex = block_scan(v, sh, &total);
     if (k < n) {
-      tcur[k] = carry + ex;
-      thist[k] = 0;
+      tcur[idx] = carry + ex;
+      thist[idx] = 0;
+      if (k % per_q == 0) {   // the queue's first word
+        tstat[kStQStart + k / per_q] = carry + ex;
+        tstat[kStQCur + (k / per_q) * kQStride] = carry + ex;
+      }
     }
     carry += total;
   }
   if (threadIdx.x == 0) {
     tstat[kStItems] = carry;
-    tstat[kStQueue] = 0;
+    tstat[kStQStart + nq] = carry;
   }
 }
 
@@ -1037,24 +1058,43 @@ struct ErodeShared {
 // read from LDS, one in an earlier window of the item from the new elevation
 // field (final: stored before the window's barrier), one in another tile from
 // the field once its mark is replaced.
+//
+// The items come from nq queues (k_deep_scan's layout); a workgroup takes
+// one item at a time from queue blockIdx.x % nq and moves to the next queue
+// only when its own is empty.  An item waits only on items of lower tiers;
+// a queue hands out its items in tier order, and every queue keeps its own
+// workgroups (nq <= the grid, all resident) until it is empty, so the
+// lowest-tier item being waited on is always held or about to be taken by
+// a running workgroup, and the grid drains.
 template <int NMODE>
 __global__ __launch_bounds__(kEroBlock) void k_deep_erode(
     GraphWork w, const uint8_t *__restrict__ rcv0, const uint32_t *__restrict__ cnt0,
     const uint32_t *__restrict__ cells, const uint64_t *__restrict__ items,
     const double *__restrict__ ftab0, const double *__restrict__ h_old0, double *h_new0,
-    int32_t *tstat) {
+    int32_t *tstat, int nq) {
   __shared__ ErodeShared sh;
-  const int nitems = tstat[kStItems];
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  int next = 0, last = 0;   // this workgroup's batch of items [next, last)
+  int q = static_cast<int>(blockIdx.x) % nq, empty = 0;   // thread 0's queue
   for (;;) {
     if (threadIdx.x == 0) {
-      if (next == last) {
-        next = atomicAdd(&tstat[kStQueue], kBatch);
-        last = min(next + kBatch, nitems);
+      uint64_t it = ~uint64_t{0};
+      while (empty < nq) {
+        int32_t *const cur = &tstat[kStQCur + q * kQStride];
+        const int end = tstat[kStQStart + q + 1];
+        // another queue is looked at before it is claimed from: near the
+        // end most of them are empty
+        if (empty == 0 ||
+            __hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < end) {
+          const int p = atomicAdd(cur, 1);
+          if (p < end) {
+            it = items[p];
+            break;
+          }
+        }
+        q = q + 1 == nq ? 0 : q + 1;
+        empty++;
       }
-      sh.item = next < last ? items[next] : ~uint64_t{0};
-      next++;
+      sh.item = it;
     }
     __syncthreads();
     const uint64_t item = sh.item;
@@ -1147,7 +1187,15 @@ int erode_grid(int ncu) {
 }
 
 __global__ void k_deep_clear(int32_t *tstat) {
-  if (threadIdx.x < kStWords) tstat[threadIdx.x] = 0;
+  if (threadIdx.x < kStClear) tstat[threadIdx.x] = 0;
+}
+
+// Queues for a persistent erosion grid of `blocks` workgroups: the largest
+// power of two up to kShards that leaves every queue a workgroup.
+inline int erode_queues(int blocks) {
+  int q = kShards;
+  while (q > 1 && q > blocks) q >>= 1;
+  return q;
 }
 
 }  // namespace deep

@@ -361,7 +361,7 @@ struct fs_ensemble {
   uint16_t *node_tier = nullptr;
   uint32_t *cells_list = nullptr;
   int32_t *thist = nullptr, *tcur = nullptr, *tstat = nullptr;
-  int graph_blocks = 0;
+  int graph_blocks = 0, graph_queues = 1;
   uint64_t graph_tiles = 0;                   // tile-graph tiles stepped so far
   std::map<int, int32_t *> recv_first;   // per recv_rows
   int cur = 0;
@@ -481,7 +481,7 @@ int graph_launches(fs_ensemble *e, const deep::GraphWork &w, int T) {
   hipLaunchKernelGGL(deep::k_deep_tiers, dim3(T), dim3(deep::kNodeCap), 0, s, w, e->node_parent,
                      e->nexit, e->node_tier, e->node_up, e->node_level, e->thist, e->tstat);
   hipLaunchKernelGGL(deep::k_deep_scan, dim3(1), dim3(deep::kBlock), 0, s, e->thist, e->tcur,
-                     e->tstat);
+                     e->tstat, e->graph_queues);
   hipLaunchKernelGGL(deep::k_deep_sum, dim3(T), dim3(deep::kNodeCap), 0, s, w, e->node_word,
                      e->node_kids, e->node_parent, e->nexit, e->node_out, e->tstat);
   hipLaunchKernelGGL(deep::k_deep_levels, dim3(T), dim3(deep::kBlock), 0, s, w, e->dinfo, e->link,
@@ -491,7 +491,7 @@ int graph_launches(fs_ensemble *e, const deep::GraphWork &w, int T) {
                      e->tile_n0, e->cells_list, e->tcur, e->items, e->ftab, h_old, h_new);
   hipLaunchKernelGGL(deep::k_deep_erode<NM>, dim3(static_cast<unsigned>(e->graph_blocks)),
                      dim3(deep::kEroBlock), 0, s, w, e->rcv, e->cnt, e->cells_list, e->items,
-                     e->ftab, h_old, h_new, e->tstat);
+                     e->ftab, h_old, h_new, e->tstat, e->graph_queues);
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) {
     fs::set_last_error(std::string("tile-graph launch: ") + hipGetErrorString(err));
@@ -919,6 +919,7 @@ int fs_ensemble_create(const fs_realization *reals, int count, int device, fs_en
   e->graph_blocks = e->nm_all == 1   ? deep::erode_grid<1>(ncu)
                     : e->nm_all == 2 ? deep::erode_grid<2>(ncu)
                                      : deep::erode_grid<-1>(ncu);
+  e->graph_queues = deep::erode_queues(e->graph_blocks);
   const int32_t *rf = nullptr;
   FS_TRY(recv_table(e, e->sched.recv_rows, &rf));
   FS_HIP_TRY(hipDeviceSynchronize());
