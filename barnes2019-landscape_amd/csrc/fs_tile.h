@@ -20,7 +20,7 @@ constexpr int kTile = 64;
 constexpr int kHalo = FS_TILE_HALO;
 constexpr int kReg = kTile + 2 * kHalo;       // region side
 constexpr int kRegCells = kReg * kReg;
-constexpr int kAccBlock = 1024;
+constexpr int kAccBlock = kHalo <= 16 ? 512 : 1024;   // 43 KB of LDS: three 512-thread tiles per CU
 constexpr int kEroBlock = 1024;
 constexpr uint8_t kOutside = 0xFD;            // region cell beyond the grid
 constexpr uint8_t kEscaped = 0xFF;            // chain leaves the region
@@ -299,7 +299,6 @@ struct AccShared {
   uint32_t scan[1];               // number of marks
   uint32_t nlist;                 // marked cells appended to hlist
   uint16_t hlist[kHaloCap];       // marked cells in marking order
-  uint8_t lv_t[kTileCells];       // level of each tile cell, stored with the counts
   int over, maxlev;
   int hist[kHalo + 1];
 };
@@ -334,7 +333,7 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
                                              int tiles_x, int32_t *level_hist,
                                              int32_t *max_level,
                                              unsigned long long *phase = nullptr) {
-  static_assert(BLOCK == kAccBlock, "shared layout is sized for kAccBlock");
+  static_assert(kRegCells / 32 <= BLOCK && kHalo + 1 <= BLOCK, "one mark word per thread");
   uint32_t *w_s = sh.w_s;
   unsigned long long *const ph = phase_row(phase);
   const uint64_t t0 = ph ? wall_clock64() : 0;
@@ -364,7 +363,7 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
   // the complement of the address of the cell that stopped it, so whether it
   // reached an outlet is read back from that cell's code afterwards.
   constexpr int kPer = kRegCells / BLOCK;
-  static_assert(kPer <= 16, "walkers per thread");
+  static_assert(kPer <= 32, "walkers per thread (bits of live)");
   int xa[kPer], hops[kPer];
   uint32_t live = 0;
 #pragma unroll
@@ -416,7 +415,9 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
       continue;
     if (x0 + tx >= W || y0 + ty >= H) continue;     // beyond the grid
     const bool rooted = xa[j] < 0 && acc_code(w_s[~xa[j] >> 2]) == kStopRoot;
-    sh.lv_t[ty * kTile + tx] = static_cast<uint8_t>(hops[j]);   // a level when rooted
+    // a level when rooted; stored straight away (a wave's cells are a run of
+    // region cells, so the bytes mostly share lines)
+    lvl[static_cast<size_t>(y0 + ty) * W + x0 + tx] = static_cast<uint8_t>(hops[j]);
     if (rooted) {
       if (level_hist) atomicAdd(&sh.hist[hops[j]], 1);
       deepest = max(deepest, hops[j]);
@@ -521,7 +522,6 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
     if (gx >= W || gy >= H) continue;
     const size_t gi = static_cast<size_t>(gy) * W + gx;
     cnt[gi] = w_s[(ty + kHalo) * kReg + tx + kHalo] & kAccCount;
-    lvl[gi] = sh.lv_t[i];
   }
   if (ph) {
     __syncthreads();
