@@ -81,6 +81,7 @@ struct fs_model {
   int W = 0, H = 0;
   int32_t N = 0;
   int device = 0;
+  int ncu = 0;                     // the device's CUs (read on first use)
   fs_params p{};
   double d_card = 0, d_diag = 0;
   ErodeConst ek{};
@@ -702,9 +703,22 @@ int tile_attempt(fs_model *m, int halo, int tiles, int tiles_x, bool clear, hipE
   // one step in kPhaseEvery (and the first of each call) runs the timed
   // instantiations that sample the phase ticks of the stage timers
   const bool timed = m->phase && (m->steps_in_call == 0 || m->tile_steps % kPhaseEvery == 0);
+  if (!m->ncu)
+    FS_HIP_TRY(hipDeviceGetAttribute(&m->ncu, hipDeviceAttributeMultiprocessorCount, m->device));
   FS_TRY(kt_begin(m, FS_K_TILE_ACC));
   if (halo == kHalo) {
-    if (timed)
+    // fewer tiles than four per CU: one 1024-thread tile per CU finishes
+    // sooner than a 512-thread one
+    const bool wide = tiles < 4 * m->ncu;
+    if (wide && timed)
+      hipLaunchKernelGGL((k_tile_acc<true, kAccBlockWide>), dim3(tiles), dim3(kAccBlockWide), 0, s,
+                         m->rcv, m->cnt, m->lvl, m->plan, m->hdr, m->W, m->H, tiles_x, tiles,
+                         m->phase);
+    else if (wide)
+      hipLaunchKernelGGL((k_tile_acc<false, kAccBlockWide>), dim3(tiles), dim3(kAccBlockWide), 0,
+                         s, m->rcv, m->cnt, m->lvl, m->plan, m->hdr, m->W, m->H, tiles_x, tiles,
+                         m->phase);
+    else if (timed)
       hipLaunchKernelGGL(k_tile_acc<true>, dim3(tiles), dim3(kAccBlock), 0, s, m->rcv, m->cnt,
                          m->lvl, m->plan, m->hdr, m->W, m->H, tiles_x, tiles, m->phase);
     else

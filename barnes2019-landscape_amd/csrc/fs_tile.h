@@ -20,7 +20,11 @@ constexpr int kTile = 64;
 constexpr int kHalo = FS_TILE_HALO;
 constexpr int kReg = kTile + 2 * kHalo;       // region side
 constexpr int kRegCells = kReg * kReg;
-constexpr int kAccBlock = kHalo <= 16 ? 512 : 1024;   // 43 KB of LDS: three 512-thread tiles per CU
+// tile accumulation: 512-thread workgroups, four tiles per CU (39 KB of LDS
+// each) with the 16-cell halo; 1024 threads where a grid has too few tiles
+// to give every CU four
+constexpr int kAccBlock = kHalo <= 16 ? 512 : 1024;
+constexpr int kAccBlockWide = 1024;
 constexpr int kEroBlock = 1024;
 constexpr uint8_t kOutside = 0xFD;            // region cell beyond the grid
 constexpr uint8_t kEscaped = 0xFF;            // chain leaves the region
@@ -538,8 +542,8 @@ __device__ __forceinline__ int tile_acc_body(AccShared &sh, const uint8_t *__res
 
 // kTimed: this launch records phase ticks (the host runs it on a sample of
 // steps; the untimed instantiation carries no timing code at all).
-template <bool kTimed>
-__global__ __launch_bounds__(kAccBlock) void k_tile_acc(const uint8_t *__restrict__ rcv,
+template <bool kTimed, int BLOCK = kAccBlock>
+__global__ __launch_bounds__(BLOCK) void k_tile_acc(const uint8_t *__restrict__ rcv,
                                                         uint32_t *__restrict__ cnt,
                                                         uint8_t *__restrict__ lvl,
                                                         TilePlan *__restrict__ plan, int32_t *hdr,
@@ -549,7 +553,7 @@ __global__ __launch_bounds__(kAccBlock) void k_tile_acc(const uint8_t *__restric
   const int t = xcd_tile(blockIdx.x, ntiles);
   // with the deep halo, hdr[0] (cleared by k_fill_counts) receives the
   // deepest tile-cell level, so the host can step back to the 16-cell halo
-  const int over = tile_acc_body<kAccBlock>(sh, rcv, cnt, lvl, plan + t, W, H, t, tiles_x, nullptr,
+  const int over = tile_acc_body<BLOCK>(sh, rcv, cnt, lvl, plan + t, W, H, t, tiles_x, nullptr,
                                             kHalo > 16 ? hdr : nullptr,
                                             kTimed ? phase : nullptr);
   if (threadIdx.x == 0 && over) atomicAdd(&hdr[3], 1);
