@@ -787,6 +787,32 @@ __global__ __launch_bounds__(kCntBlock) __attribute__((amdgpu_waves_per_eu(6))) 
   const int t = wt.tg;   // per-tile arrays and node ids
   const Box b = tile_box(wt.t, G.tiles_x, W, H);
   const int n0 = uni(tile_n0[t]) & 0xFFFF;
+  // the ring cell this thread looks after for the inflows: its receiver
+  // byte is requested before the tile's loads, its link and count after
+  // them, so the three dependent reads overlap the tile's
+  int ring_gx = 0, ring_gy = 0;
+  uint8_t ring_r = 0xFF;
+  {
+    const int q = threadIdx.x;
+    const int ring_top = b.tw + 2, nring = 2 * ring_top + 2 * b.th;
+    if (q < nring) {
+      if (q < ring_top) {
+        ring_gx = b.x0 - 1 + q;
+        ring_gy = b.y0 - 1;
+      } else if (q < 2 * ring_top) {
+        ring_gx = b.x0 - 1 + (q - ring_top);
+        ring_gy = b.y0 + b.th;
+      } else if (q < 2 * ring_top + b.th) {
+        ring_gx = b.x0 - 1;
+        ring_gy = b.y0 + (q - 2 * ring_top);
+      } else {
+        ring_gx = b.x0 + b.tw;
+        ring_gy = b.y0 + (q - 2 * ring_top - b.th);
+      }
+      if (ring_gx >= 0 && ring_gy >= 0 && ring_gx < W && ring_gy < H)
+        ring_r = rcv[static_cast<size_t>(ring_gy) * W + ring_gx];
+    }
+  }
   uint32_t info[kCntPer];
   if (threadIdx.x == 0) sh.ntier = sh.maxhops = 0;
   if (threadIdx.x < kTiersPerTile) S.hist[threadIdx.x] = 0;
@@ -804,6 +830,16 @@ __global__ __launch_bounds__(kCntBlock) __attribute__((amdgpu_waves_per_eu(6))) 
       sh.cs[i] = cnt[g];
     }
     sh.code[i] = c;
+  }
+  int ring_cell = -1;   // the tile cell the ring cell drains into
+  uint32_t ring_v = 0;  // and the count through the ring cell
+  if (ring_r < 8) {
+    const int tx = ring_gx + dir_dx(ring_r) - b.x0, ty = ring_gy + dir_dy(ring_r) - b.y0;
+    if (static_cast<unsigned>(tx) < static_cast<unsigned>(b.tw) &&
+        static_cast<unsigned>(ty) < static_cast<unsigned>(b.th)) {
+      ring_cell = ty * kSide + tx;
+      ring_v = node_out[link[static_cast<size_t>(ring_gy) * W + ring_gx]];
+    }
   }
   const int n = nexit[t];
   int my_tier = -1;
@@ -831,42 +867,13 @@ __global__ __launch_bounds__(kCntBlock) __attribute__((amdgpu_waves_per_eu(6))) 
   // inflows: a ring cell that drains into the tile is an exit of its own
   // tile; its count is added along the in-tile chain from the cell it
   // drains into
-  {
-    const int q = threadIdx.x;
-    const int ring_top = b.tw + 2, nring = 2 * ring_top + 2 * b.th;
-    if (q < nring) {
-      int gx, gy;
-      if (q < ring_top) {
-        gx = b.x0 - 1 + q;
-        gy = b.y0 - 1;
-      } else if (q < 2 * ring_top) {
-        gx = b.x0 - 1 + (q - ring_top);
-        gy = b.y0 + b.th;
-      } else if (q < 2 * ring_top + b.th) {
-        gx = b.x0 - 1;
-        gy = b.y0 + (q - 2 * ring_top);
-      } else {
-        gx = b.x0 + b.tw;
-        gy = b.y0 + (q - 2 * ring_top - b.th);
-      }
-      if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
-        const size_t g = static_cast<size_t>(gy) * W + gx;
-        const uint8_t r = rcv[g];
-        if (r < 8) {
-          const int tx = gx + dir_dx(r) - b.x0, ty = gy + dir_dy(r) - b.y0;
-          if (static_cast<unsigned>(tx) < static_cast<unsigned>(b.tw) &&
-              static_cast<unsigned>(ty) < static_cast<unsigned>(b.th)) {
-            const uint32_t v = node_out[link[g]];
-            int cur = ty * kSide + tx;
-            for (int hop = 0; hop < kCells; hop++) {
-              atomicAdd(&sh.cs[cur], v);
-              const int c = sh.code[cur] & kCodeStep;
-              if (c >= 8) break;
-              cur = tile_step(cur, c);
-            }
-          }
-        }
-      }
+  if (ring_cell >= 0) {
+    int cur = ring_cell;
+    for (int hop = 0; hop < kCells; hop++) {
+      atomicAdd(&sh.cs[cur], ring_v);
+      const int c = sh.code[cur] & kCodeStep;
+      if (c >= 8) break;
+      cur = tile_step(cur, c);
     }
   }
   __syncthreads();
