@@ -277,8 +277,10 @@ __device__ __forceinline__ void receivers_strips(const double *__restrict__ h,
 }
 
 // Also clears the header words the tile path's accumulation reports into
-// (hdr[0], hdr[1], hdr[3]; as k_fill_counts with no level counts).
-__global__ __launch_bounds__(256) void k_receivers(const double *__restrict__ h,
+// (hdr[0], hdr[1], hdr[3]; as k_fill_counts with no level counts).  Compiled
+// for eight waves per SIMD (its waves wait on memory most of their cycles;
+// 106 scalar registers held it to fewer).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_receivers(const double *__restrict__ h,
                                                    uint8_t *__restrict__ rcv, int W, int H,
                                                    double d_card, double d_diag,
                                                    int32_t *__restrict__ hdr) {
