@@ -337,17 +337,22 @@ __global__ __launch_bounds__(kLocalBlock) void k_deep_local(
     if (threadIdx.x == 0) G.hdr[0] = 0;
     for (int q = threadIdx.x; q < G.nlev; q += kLocalBlock) G.levels[q] = 0;
   }
-  uint8_t rr[kLocalPer];   // receiver bytes, kept for the exits' marks
+  // receiver bytes, kept for the exits' marks; all requested before any is
+  // used (a cell past a partial tile's edge reads one inside it)
+  uint8_t rr[kLocalPer];
+#pragma unroll
+  for (int j = 0; j < kLocalPer; j++) {
+    const int i = j * kLocalBlock + threadIdx.x;
+    const int lx = min(i % kSide, b.tw - 1), ly = min(i / kSide, b.th - 1);
+    rr[j] = rcv[static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx];
+  }
 #pragma unroll
   for (int j = 0; j < kLocalPer; j++) {
     const int i = j * kLocalBlock + threadIdx.x;
     const int lx = i % kSide, ly = i / kSide;
     uint16_t n = kTermNone;
-    rr[j] = 0xFF;
-    if (lx < b.tw && ly < b.th) {
-      rr[j] = rcv[static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx];
-      n = tile_next(rr[j], lx, ly, b);
-    }
+    if (lx < b.tw && ly < b.th) n = tile_next(rr[j], lx, ly, b);
+    else rr[j] = 0xFF;
     nx[i] = n;
     jw[i] = n < kTermNone ? (static_cast<uint32_t>(n) | (1u << 16)) : static_cast<uint32_t>(i);
     lc[i] = n != kTermNone ? 1 : 0;
@@ -817,19 +822,30 @@ __global__ __launch_bounds__(kCntBlock) __attribute__((amdgpu_waves_per_eu(6))) 
   if (threadIdx.x == 0) sh.ntier = sh.maxhops = 0;
   if (threadIdx.x < kTiersPerTile) S.hist[threadIdx.x] = 0;
   for (int q = threadIdx.x; q < kHashCap; q += kCntBlock) S.tset[q] = -1;
+  // every load is issued before any is used (a cell past a partial tile's
+  // edge reads a cell inside it and ignores the value): a branch per cell
+  // would make each cell's loads wait on the previous cell's
+  uint8_t rv[kCntPer];
+  uint32_t cv[kCntPer];
+#pragma unroll
+  for (int j = 0; j < kCntPer; j++) {
+    const int i = j * kCntBlock + threadIdx.x;
+    const int lx = min(i % kSide, b.tw - 1), ly = min(i / kSide, b.th - 1);
+    const size_t g = static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx;
+    rv[j] = rcv[g];
+    info[j] = dinfo[g];
+    cv[j] = cnt[g];
+  }
 #pragma unroll
   for (int j = 0; j < kCntPer; j++) {
     const int i = j * kCntBlock + threadIdx.x;
     const int lx = i % kSide, ly = i / kSide;
-    uint8_t c = kCodeNone;
-    info[j] = 0;
-    if (lx < b.tw && ly < b.th) {
-      const size_t g = static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx;
-      info[j] = dinfo[g];
-      c = tile_code(rcv[g], lx, ly, b) | ((info[j] & kInfoExit) ? kCodeExitTail : 0);
-      sh.cs[i] = cnt[g];
-    }
-    sh.code[i] = c;
+    const bool in = lx < b.tw && ly < b.th;
+    if (!in) info[j] = 0;
+    sh.code[i] = in ? static_cast<uint8_t>(tile_code(rv[j], lx, ly, b) |
+                                            ((info[j] & kInfoExit) ? kCodeExitTail : 0))
+                    : kCodeNone;
+    sh.cs[i] = in ? cv[j] : 0u;
   }
   int ring_cell = -1;   // the tile cell the ring cell drains into
   uint32_t ring_v = 0;  // and the count through the ring cell
