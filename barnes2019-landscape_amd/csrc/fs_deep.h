@@ -1001,8 +1001,11 @@ __global__ __launch_bounds__(kCntBlock) __attribute__((amdgpu_waves_per_eu(6))) 
 #pragma unroll
   for (int c = 0; c < kCntPer; c++) {   // kCntWaves waves cover kCells in kCntPer rounds
     const int p = (wave + c * kCntWaves) * kWave + lane;
-    ci[c] = p < n0 ? static_cast<int>(list[p] & kEntCell) : -1;
+    ci[c] = static_cast<int>(list[p] & kEntCell);   // read whatever p: one wait for all
   }
+#pragma unroll
+  for (int c = 0; c < kCntPer; c++)
+    if ((wave + c * kCntWaves) * kWave + lane >= n0) ci[c] = -1;
 #pragma unroll
   for (int c = 0; c < kCntPer; c++)
     if (ci[c] >= 0)
