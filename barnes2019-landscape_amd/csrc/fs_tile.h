@@ -600,8 +600,7 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
                                                 const double *__restrict__ ftab, int W, int H,
                                                 int t, int tiles_x, const ErodeConst &k,
                                                 int32_t *hdr,
-                                                unsigned long long *phase = nullptr,
-                                                const int32_t *skip_if = nullptr) {
+                                                unsigned long long *phase = nullptr) {
   constexpr int kWaves = kEroBlock / kWave;
   unsigned long long *const ph = phase_row(phase);
   const uint64_t t0 = ph ? wall_clock64() : 0;
@@ -621,11 +620,10 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
   const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
   const int rx0 = x0 - kHalo, ry0 = y0 - kHalo;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  for (int i = threadIdx.x; i < kKeys * kWaves; i += kEroBlock) (&wcount[0][0])[i] = 0;
+  if (threadIdx.x < 2 * kEroFactCache) ft_s[threadIdx.x] = ftab[threadIdx.x];
   // the plan and the tile's elevations, counts and levels; a halo cell's
-  // elevation and count as soon as its region index arrives.  Everything is
-  // requested before anything waits: the factor cache, the skip flag and the
-  // LDS writes come after the tile's loads are in flight
-  const double ftv = threadIdx.x < 2 * kEroFactCache ? ftab[threadIdx.x] : 0.0;
+  // elevation and count as soon as its region index arrives
   const int nhalo = plan->nhalo;
   const int nh = nhalo < 0 ? 0 : (nhalo > kHaloCap ? kHaloCap : nhalo);
   {
@@ -654,10 +652,6 @@ __device__ __forceinline__ void tile_erode_body(EroShared &sh, const double *__r
     const size_t hg = static_cast<size_t>(ry0 + hr / kReg) * W + (rx0 + hr % kReg);
     const double hh = halo ? h_old[hg] : 0.0;
     const uint32_t hc = halo ? cnt[hg] : 0u;
-    // a grid whose tiles were too deep does nothing (its plans are incomplete)
-    if (skip_if && *skip_if != 0) return;
-    for (int i = threadIdx.x; i < kKeys * kWaves; i += kEroBlock) (&wcount[0][0])[i] = 0;
-    if (threadIdx.x < 2 * kEroFactCache) ft_s[threadIdx.x] = ftv;
 #pragma unroll
     for (int j = 0; j < kTilePer; j++) {
       hs[j * kEroBlock + threadIdx.x] = th[j];
@@ -821,8 +815,9 @@ __global__ __launch_bounds__(kEroBlock) void k_tile_erode(
   // launched right behind the accumulation, before the host has read its
   // outcome: when a tile was too deep (hdr[3], the plans are incomplete)
   // the whole grid does nothing
+  if (hdr[3] != 0) return;
   const int t = xcd_tile(blockIdx.x, ntiles);
   tile_erode_body<NMODE>(sh, h_old, h_new, plan + t, lvl, cnt, ftab, W, H, t, tiles_x, k, hdr,
-                         kTimed ? phase : nullptr, &hdr[3]);
+                         kTimed ? phase : nullptr);
 }
 
