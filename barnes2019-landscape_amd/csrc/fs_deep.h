@@ -314,11 +314,9 @@ __global__ __launch_bounds__(kLocalBlock) void k_deep_local(
     int32_t *__restrict__ nexit, int32_t *__restrict__ tile_n0, uint32_t *__restrict__ cells,
     double *__restrict__ h_new0, int32_t *__restrict__ tstat) {
   __shared__ uint16_t nx[kCells];
-  __shared__ uint32_t jw[kCells];    // pointer jumping: ancestor | hops to it << 16
-  // local counts: at most kCells, so 16 bits; added through the 32-bit word
-  // that holds a pair (no carry crosses halves), which keeps the tile's LDS
-  // small enough for four workgroups per CU
-  __shared__ __attribute__((aligned(4))) uint16_t lc[kCells];
+  // walk words: (next cell + 1) << 16 | local count; 0 in the top half at a
+  // tail (later the root list's counters)
+  __shared__ uint32_t jw[kCells];
   __shared__ uint16_t slot_node[4 * kSide];   // node number of the exit at each border position
   __shared__ int wave_nodes[4 * kSide / kWave];
   __shared__ int maxhops, scan[kLocalBlock / kWave + 1];
@@ -354,56 +352,42 @@ __global__ __launch_bounds__(kLocalBlock) void k_deep_local(
     if (lx < b.tw && ly < b.th) n = tile_next(rr[j], lx, ly, b);
     else rr[j] = 0xFF;
     nx[i] = n;
-    jw[i] = n < kTermNone ? (static_cast<uint32_t>(n) | (1u << 16)) : static_cast<uint32_t>(i);
-    lc[i] = n != kTermNone ? 1 : 0;
+    jw[i] = n < kTermNone ? (static_cast<uint32_t>(n) + 1u) << 16 : 0u;
   }
   __syncthreads();
-  // tail and hops of every cell by pointer jumping: each round doubles the
-  // hops a pointer covers (reading a word another thread rewrote in the
-  // same round is still a valid ancestor and distance)
-  bool bad = true;
-  for (int round = 0; round <= 12; round++) {
-    bool moved = false;
-#pragma unroll
-    for (int j = 0; j < kLocalPer; j++) {
-      const int i = j * kLocalBlock + threadIdx.x;
-      const uint32_t w = jw[i];
-      const int a = static_cast<int>(w & 0xFFFFu);
-      if (a == i) continue;
-      const uint32_t wa = jw[a];
-      if (static_cast<int>(wa & 0xFFFFu) == a) continue;
-      jw[i] = (wa & 0xFFFFu) | ((w >> 16) + (wa >> 16)) << 16;
-      moved = true;
-    }
-    if (!__syncthreads_or(moved)) {
-      bad = false;
-      break;
-    }
-  }
-  if (bad && threadIdx.x == 0) atomicOr(&tstat[kStErr], kErrChain);   // a receiver cycle
+  // every tile cell walks its in-tile chain to its tail, adding 1 to each
+  // cell it passes, itself included: one LDS atomic per step returns the
+  // cell's word, whose top half names the next cell.  The walk gives the
+  // cell's tail and hops, the adds every cell's local count (the tile cells
+  // upstream of it, itself included).  A thread's walks advance together,
+  // one step per round, with no barrier.
   int cur[kLocalPer], hops[kLocalPer];
-  int mh = 0;
+  uint32_t alive = 0;
 #pragma unroll
   for (int j = 0; j < kLocalPer; j++) {
-    const uint32_t w = jw[j * kLocalBlock + threadIdx.x];
-    cur[j] = static_cast<int>(w & 0xFFFu);
-    hops[j] = static_cast<int>(w >> 16) & 0xFFF;
-    mh = max(mh, hops[j]);
+    const int i = j * kLocalBlock + threadIdx.x;
+    cur[j] = i;
+    hops[j] = 0;
+    if (i % kSide < b.tw && i / kSide < b.th) alive |= 1u << j;
   }
-  const int top = block_max(mh, &maxhops);
-  // local counts from the top hops down: a cell's in-tile donors are one hop
-  // further from the tail, so their counts are complete when it adds its own
-  for (int L = top; L >= 1; L--) {
+  for (int step = 0; alive != 0 && step < kCells; step++) {
 #pragma unroll
-    for (int j = 0; j < kLocalPer; j++)
-      if (hops[j] == L) {
-        const int i = j * kLocalBlock + threadIdx.x;
-        const int d = nx[i];
-        atomicAdd(reinterpret_cast<uint32_t *>(lc) + (d >> 1),
-                  static_cast<uint32_t>(lc[i]) << ((d & 1) * 16));
+    for (int j = 0; j < kLocalPer; j++) {
+      if (!(alive & (1u << j))) continue;
+      const uint32_t nxt = atomicAdd(&jw[cur[j]], 1u) >> 16;
+      if (nxt == 0) {
+        alive &= ~(1u << j);
+      } else {
+        cur[j] = static_cast<int>(nxt) - 1;
+        hops[j]++;
       }
-    __syncthreads();
+    }
   }
+  if (alive != 0) atomicOr(&tstat[kStErr], kErrChain);   // a receiver cycle
+  int mh = 0;
+#pragma unroll
+  for (int j = 0; j < kLocalPer; j++) mh = max(mh, hops[j]);
+  const int top = block_max(mh, &maxhops);   // its barrier ends every walk
   // exits become the tile's nodes, numbered in border-position order: the
   // first four waves take one border position per lane
   constexpr int kSlotWaves = 4 * kSide / kWave;
@@ -438,7 +422,7 @@ __global__ __launch_bounds__(kLocalBlock) void k_deep_local(
       slot_node[threadIdx.x] = static_cast<uint16_t>(k);
       const int id = t * kNodeCap + k;
       node_cell[id] = (b.y0 + slot_cell / kSide) * W + b.x0 + slot_cell % kSide;
-      node_word[id] = lc[slot_cell];
+      node_word[id] = jw[slot_cell] & 0xFFFFu;
       node_kids[id] = 0;
     }
   }
@@ -450,7 +434,7 @@ __global__ __launch_bounds__(kLocalBlock) void k_deep_local(
     const int lx = i % kSide, ly = i / kSide;
     const size_t g = static_cast<size_t>(b.y0 + ly) * W + b.x0 + lx;
     const bool exit_tail = nx[cur[j]] == kTermExit;
-    cnt[g] = lc[i];
+    cnt[g] = jw[i] & 0xFFFFu;
     dinfo[g] = static_cast<uint32_t>(cur[j]) | (static_cast<uint32_t>(hops[j]) << 12) |
                (exit_tail ? kInfoExit : 0u);
     if (lx == 0 || ly == 0 || lx == b.tw - 1 || ly == b.th - 1)
@@ -461,7 +445,9 @@ __global__ __launch_bounds__(kLocalBlock) void k_deep_local(
   }
   if (threadIdx.x == 0) nexit[t] = nnode;   // every slot thread summed the same count
   // the cells with a root tail, sorted by hops, start the tile's cell list
-  // (jw is free now: its words count the cells per hops)
+  // (jw is free once every count is stored: its words count the cells per
+  // hops)
+  __syncthreads();
   for (int q = threadIdx.x; q <= top; q += kLocalBlock) jw[q] = 0;
   __syncthreads();
   bool rooted[kLocalPer];
